@@ -1,0 +1,210 @@
+/*
+ * pvvc_hip.h -- C ABI of the MI355X-native patchVVC per-patch GoP decoder.
+ *
+ * Drop-in surface: the reference client's decode-frame entry
+ * (`launch_cudaProcess`, include/cuda/render.cuh:117-130, defined in
+ * src/cuda/render/CUDA2OpenGL.cu:68-74) and its frame loader
+ * (`io::LoadFrame`, src/cuda/common/slice_io.cu:143-275), plus native
+ * status-returning calls for the batched persistent decode and the
+ * multi-stream serving loop.  All device pointers are plain `void*`/typed
+ * pointers into HIP device memory; streams are passed as `void*`
+ * (a hipStream_t).  No call allocates from the device heap.
+ */
+#ifndef PVVC_HIP_H_
+#define PVVC_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes.  Per-slice codes follow the reference CPU decoder's         */
+/* exceptions (src/octree/invert_raht_octree.cc:25-40,188-190,264-266;       */
+/* src/common/entropy_codec.cc:345-353) instead of aborting the process.     */
+/* ------------------------------------------------------------------------ */
+enum pvvc_status {
+    PVVC_OK = 0,
+    PVVC_ERR_BAD_SLICE = 1,        /* invalid type bit, no data, short geometry, height 0 or > 64 */
+    PVVC_ERR_EMPTY_REFERENCE = 2,  /* predictive slice without a decoded intra slice */
+    PVVC_ERR_UNMATCHED_SIZE = 3,   /* leaf count != size, or P size != intra size */
+    PVVC_ERR_ZSTD = 4,             /* zstd frame could not be unwrapped */
+    PVVC_ERR_BITSTREAM = 5,        /* RLGR read past the colour bytes, or run exponent > 30 */
+    PVVC_ERR_SYMBOL_RANGE = 6,     /* RLGR symbol outside int32 */
+    PVVC_ERR_CAPACITY = 7,         /* patch larger than its state slot / workspace */
+    PVVC_ERR_INVALID_ARG = 8,
+    PVVC_ERR_HIP = 9,              /* a HIP runtime call failed */
+    PVVC_ERR_TIMEOUT = 10,         /* waited too long for the intra slice of the same patch */
+    PVVC_ERR_IO = 11,              /* file error (see pvvc_load_frame for the reference codes) */
+    PVVC_ERR_BUSY = 12             /* serving: no slot / memory to admit a stream */
+};
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in decode-frame entry                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Output point, identical to the reference's `Points`
+ * (include/cuda/base.cuh:50-58): xyz after the patch's rigid motion, RGB as
+ * unrounded float / 255 (src/cuda/render/CUDA2OpenGL.cu:46-51). */
+typedef struct Points {
+    float x, y, z;
+    float r, g, b;
+} Points;
+
+/* Opaque device-resident per-patch decoder state: replaces the array of
+ * `vvc::client::octree::InvertRAHTOctree` objects
+ * (include/cuda/octree.cuh:413-454) that the reference caller allocates
+ * with cudaMalloc and fills with default-constructed host copies
+ * (src/cuda/common/manager.cc:235-238, test/test_decode_only.cc:51-55).
+ * Holds, per patch index, the intra geometry bytes, the reference colours
+ * and a decode workspace; nothing is taken from the device heap. */
+typedef struct pvvc_state pvvc_state;
+
+/* Allocates state for patch indices [0, n_patches) with up to max_points
+ * points per patch.  Returns NULL on failure (pvvc_last_error() says why). */
+pvvc_state* pvvc_state_alloc(int n_patches, int max_points);
+void pvvc_state_free(pvvc_state* state);
+/* Status of patch `patch` from the last decode that touched it (D2H read,
+ * synchronises the library stream). */
+int pvvc_state_patch_status(const pvvc_state* state, int patch);
+
+/* Drop-in for launch_cudaProcess (include/cuda/render.cuh:117-130,
+ * src/cuda/render/CUDA2OpenGL.cu:68-74).  Same argument meaning: every
+ * pointer is caller-owned device memory; geometry[s]/color[s] hold the
+ * already zstd-unwrapped bytes of slice s; mv[s] is 16 row-major floats;
+ * inner_offset[s] is the output offset of slice s; the state array is
+ * indexed by index[s].  grid and block are accepted and ignored (the
+ * library picks the launch shape).  Enqueued asynchronously on the stream
+ * set by pvvc_set_stream (default: the null stream); the caller
+ * synchronises, as the reference callers do (test/test_decode_only.cc:147).
+ * The reference returns nothing and reports nothing; failures here are
+ * recorded per patch (pvvc_state_patch_status) and in pvvc_last_error(). */
+void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, int* inner_offset, int* index,
+                        uint8_t* type, float** mv, uint32_t* size, uint8_t* qp, uint8_t** geometry,
+                        uint32_t* geometry_size, uint8_t** color, uint32_t* color_size, pvvc_state* invertRAHTOctree_gpu,
+                        int patch_size);
+
+/* Stream used by launch_cudaProcess and the native calls (a hipStream_t). */
+int pvvc_set_stream(void* stream);
+/* Last error recorded by any call in this process (sticky until read). */
+int pvvc_last_error(void);
+/* Reference-only rendering artefact: patches with fewer than 10 points
+ * forced to white (src/cuda/render/CUDA2OpenGL.cu:54-58).  Off by default. */
+int pvvc_set_white_small_patches(int enable);
+
+/* ------------------------------------------------------------------------ */
+/* Container (host): replaces io::LoadFrame (src/cuda/common/slice_io.cu:    */
+/* 143-275).  The file layout is the one ChangeSliceToFrame writes           */
+/* (src/io/slice_io.cc:258-315).  Blobs flagged zstd (type bit 3 geometry,   */
+/* bit 4 colour) are unwrapped on load, as the reference loader does.        */
+/* ------------------------------------------------------------------------ */
+typedef struct pvvc_frame pvvc_frame;
+
+/* Returns 0, or the reference loader's codes: 1 wrong file name (not
+ * "*.frame"), 2 no such file, 3 permission denied, 4 other open error,
+ * 5 short read; plus 6 for a zstd frame that fails to unwrap. */
+int pvvc_load_frame(const char* path, pvvc_frame** out);
+/* Parses a .frame image already in host memory (same codes, 1-4 unused). */
+int pvvc_parse_frame(const uint8_t* bytes, size_t nbytes, pvvc_frame** out);
+void pvvc_frame_free(pvvc_frame* frame);
+int pvvc_frame_timestamp(const pvvc_frame* frame);
+uint32_t pvvc_frame_slice_cnt(const pvvc_frame* frame);
+uint64_t pvvc_frame_points(const pvvc_frame* frame);
+/* Copies per-slice metadata out (any pointer may be NULL).  mv is [S][16]. */
+void pvvc_frame_meta(const pvvc_frame* frame, int* index, uint8_t* type, uint32_t* size, uint8_t* qp, float* mv,
+                     uint32_t* geometry_size, uint32_t* color_size);
+const uint8_t* pvvc_frame_geometry(const pvvc_frame* frame, uint32_t slice);
+const uint8_t* pvvc_frame_color(const pvvc_frame* frame, uint32_t slice);
+
+/* Stages a frame for launch_cudaProcess the way test/test_decode_only.cc:
+ * 97-123 does, but with one packed host->device copy: fills the device
+ * arrays of the drop-in call.  `dev` must come from pvvc_dropin_alloc. */
+typedef struct pvvc_dropin_frame {
+    int* inner_offset;
+    int* index;
+    uint8_t* type;
+    float** mv;
+    uint32_t* size;
+    uint8_t* qp;
+    uint8_t** geometry;
+    uint32_t* geometry_size;
+    uint8_t** color;
+    uint32_t* color_size;
+    int slice_cnt;
+    int64_t point_cnt;
+    void* blob;        /* device allocation holding all of the above */
+    size_t blob_bytes;
+} pvvc_dropin_frame;
+int pvvc_dropin_stage(const pvvc_frame* frame, pvvc_dropin_frame* dev);
+void pvvc_dropin_release(pvvc_dropin_frame* dev);
+
+/* ------------------------------------------------------------------------ */
+/* Native decoder: batched persistent decode                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct pvvc_decoder pvvc_decoder;
+typedef struct pvvc_batch pvvc_batch;
+
+/* Tunables (autotuner output).  Zero fields take defaults. */
+typedef struct pvvc_tuning {
+    int waves_per_patch;   /* 1, 2 or 4 waves per work item */
+    int lds_bytes;         /* LDS workspace per work item (4..64 KiB) */
+    int wgs_per_cu;        /* persistent workgroups per CU */
+    int queue_chunk;       /* items taken per queue pop */
+} pvvc_tuning;
+
+int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_patch);
+int pvvc_decoder_destroy(pvvc_decoder* dec);
+int pvvc_decoder_set_tuning(pvvc_decoder* dec, const pvvc_tuning* tuning);
+int pvvc_decoder_get_tuning(const pvvc_decoder* dec, pvvc_tuning* tuning);
+
+/* Packs frames [0, n_frames) (in decode order) into one device-resident
+ * batch: a single host->device copy.  Work items are all slices of all
+ * frames; patches whose intra slice is in the batch are ordered after it. */
+int pvvc_batch_create(pvvc_decoder* dec, const pvvc_frame* const* frames, int n_frames, pvvc_batch** out);
+void pvvc_batch_free(pvvc_batch* batch);
+int pvvc_batch_frames(const pvvc_batch* batch);
+int64_t pvvc_batch_frame_points(const pvvc_batch* batch, int frame);
+int64_t pvvc_batch_frame_offset(const pvvc_batch* batch, int frame);
+int64_t pvvc_batch_input_bytes(const pvvc_batch* batch);
+
+/* Output modes. */
+enum pvvc_output { PVVC_OUT_NATIVE = 0, PVVC_OUT_POINTS = 1 };
+/* Decode modes. */
+enum pvvc_mode {
+    PVVC_MODE_PER_FRAME = 0,   /* one launch per frame, one workgroup per slice */
+    PVVC_MODE_PERSISTENT = 1,  /* whole batch in one persistent launch, length-sorted queue */
+    PVVC_MODE_GRAPH = 2        /* per-frame launches replayed from one captured hipGraph per frame */
+};
+
+/* Decodes the batch asynchronously on the library stream.  Output for
+ * frame f starts at point pvvc_batch_frame_offset(f).  Native output:
+ * xyz (float x3) and rgb (uint8 x3) per point; Points output: `points`. */
+int pvvc_decode_batch(pvvc_decoder* dec, pvvc_batch* batch, int mode, int output, float* xyz, uint8_t* rgb,
+                      Points* points);
+/* Per-item status of the last decode of this batch (synchronises). */
+int pvvc_batch_status(pvvc_batch* batch, int* status, int n);
+/* Resets every patch state slot (drops GoP state). */
+int pvvc_decoder_reset(pvvc_decoder* dec);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory / timing helpers (plumbing for callers without a HIP        */
+/* binding of their own).                                                    */
+/* ------------------------------------------------------------------------ */
+void* pvvc_dev_alloc(size_t bytes);
+void pvvc_dev_free(void* ptr);
+int pvvc_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int pvvc_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int pvvc_memset(void* dst, int value, size_t bytes);
+int pvvc_synchronize(void);
+/* Kernel time of the last decode call, from HIP events recorded on the
+ * library stream around its launches (milliseconds). */
+double pvvc_last_kernel_ms(void);
+/* Device properties used for launch shapes: CU count, LDS per workgroup. */
+int pvvc_device_info(int* cu_count, int* lds_per_block, int* warp_size, char* name, int name_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PVVC_HIP_H_ */

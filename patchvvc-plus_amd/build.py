@@ -1,0 +1,85 @@
+"""Builds the native libraries of the package in-tree (no cmake/ninja).
+
+  pvvc/_lib/libpvvc_hip.so  HIP kernels (gfx950) + C ABI + .frame loader
+  pvvc/_lib/libpvvc_enc.so  stream writer / synthetic generator (host C++)
+
+Run: python patchvvc-plus_amd/build.py [--jobs N]
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+OUT = HERE / "pvvc" / "_lib"
+REPO = HERE.parent
+
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
+ARCH = os.environ.get("PVVC_OFFLOAD_ARCH", "gfx950")
+
+
+def zstd_paths() -> tuple[str, str]:
+    inc = next((p for p in ("/opt/conda/include", "/usr/include") if Path(p, "zstd.h").exists()), None)
+    lib = next((p for p in ("/usr/lib/x86_64-linux-gnu/libzstd.so.1", "/opt/conda/lib/libzstd.so.1")
+                if Path(p).exists()), None)
+    if inc is None or lib is None:
+        raise RuntimeError("zstd header/library not found")
+    return inc, lib
+
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-value",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+             "-Wno-sign-compare", "-Wno-unused-function"]
+
+
+def run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} ... {cmd[-1]}")
+    if r.stderr.strip():
+        # keep warnings visible but short
+        lines = [ln for ln in r.stderr.splitlines() if "warning" in ln and "packed-fp32-ops" not in ln]
+        if lines:
+            sys.stderr.write("\n".join(lines[:20]) + "\n")
+
+
+def build(jobs: int = 4) -> list[Path]:
+    OUT.mkdir(parents=True, exist_ok=True)
+    inc, zlib = zstd_paths()
+    obj = OUT / "obj"
+    obj.mkdir(exist_ok=True)
+    steps = {
+        obj / "frame_io.o": [CXX, *CXX_FLAGS, f"-I{inc}", "-c", str(CSRC / "frame_io.cpp"), "-o", str(obj / "frame_io.o")],
+        obj / "pvvc_decode.o": [HIPCC, *HIP_FLAGS, f"-I{inc}", "-c", str(CSRC / "pvvc_decode.hip"), "-o",
+                                str(obj / "pvvc_decode.o")],
+        OUT / "libpvvc_enc.so": [CXX, *CXX_FLAGS, f"-I{inc}", "-shared", str(CSRC / "stream_writer.cpp"), "-o",
+                                 str(OUT / "libpvvc_enc.so"), zlib, "-lpthread"],
+    }
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(run, steps.values()))
+    hip_so = OUT / "libpvvc_hip.so"
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(obj / "pvvc_decode.o"), str(obj / "frame_io.o"),
+         "-o", str(hip_so), zlib, "-lpthread"])
+    shutil.rmtree(obj, ignore_errors=True)
+    return [hip_so, OUT / "libpvvc_enc.so"]
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    args = ap.parse_args()
+    for p in build(args.jobs):
+        print(p)

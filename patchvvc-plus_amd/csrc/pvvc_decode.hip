@@ -1,0 +1,1507 @@
+/*
+ * pvvc_decode.hip -- HIP kernels and C ABI of the per-patch GoP decoder.
+ *
+ * Device pipeline per slice: see pvvc_kernels.h.  Host API: include/pvvc_hip.h.
+ */
+#include "pvvc_internal.h"
+#include "pvvc_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+namespace pvvc {
+
+/* ======================================================================== */
+/* Device: RLGR decoder (src/common/entropy_codec.cc:163-283)               */
+/* ======================================================================== */
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+/* Big-endian dword at byte `off` of the colour stream; bytes past the end read 0. */
+__device__ __forceinline__ uint32_t fetch_be(const uint8_t* __restrict__ src, uint32_t nbytes, uint32_t off,
+                                             uint32_t aligned) {
+    if (aligned && off + 4u <= nbytes) return __builtin_bswap32(*reinterpret_cast<const uint32_t*>(src + off));
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) v = (v << 8) | ((off + i) < nbytes ? static_cast<uint32_t>(src[off + i]) : 0u);
+    return v;
+}
+
+/* Wave-uniform MSB-first bit reader.  `cur`/`nxt` are per-lane: dword `lane`
+ * of the current and the next 256-byte chunk. */
+struct BitReader {
+    const uint8_t* src;
+    uint32_t nbytes;
+    uint32_t aligned;
+    uint32_t cur, nxt;
+    uint32_t chunk;
+    uint32_t wpos;   /* next dword index to append to the window */
+    uint64_t win;    /* left-aligned window, nb valid bits, zero below */
+    int32_t nb;
+};
+
+__device__ __forceinline__ uint32_t br_chunk(const BitReader& br, uint32_t chunk) {
+    return fetch_be(br.src, br.nbytes, (chunk * 64u + lane_id()) * 4u, br.aligned);
+}
+
+__device__ __forceinline__ void br_init(BitReader& br, const uint8_t* src, uint32_t nbytes) {
+    br.src = src;
+    br.nbytes = nbytes;
+    br.aligned = (reinterpret_cast<uintptr_t>(src) & 3u) == 0;
+    br.cur = br_chunk(br, 0);
+    br.nxt = br_chunk(br, 1);
+    br.chunk = 0;
+    br.wpos = 0;
+    br.win = 0;
+    br.nb = 0;
+}
+
+/* Appends 32 bits when at most 32 are left, so nb > 32 afterwards. */
+__device__ __forceinline__ void br_refill(BitReader& br) {
+    if (br.nb <= 32) {
+        const uint32_t l = br.wpos & 63u;
+        if (l == 0 && br.wpos != 0) {
+            br.cur = br.nxt;
+            br.chunk += 1;
+            br.nxt = br_chunk(br, br.chunk + 1);
+        }
+        const uint32_t w = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(br.cur), static_cast<int>(l)));
+        br.win |= static_cast<uint64_t>(w) << (32 - br.nb);
+        br.nb += 32;
+        br.wpos += 1;
+    }
+}
+
+__device__ __forceinline__ void br_skip(BitReader& br, uint32_t b) {
+    br.win = b >= 64 ? 0ull : (br.win << b);
+    br.nb -= static_cast<int32_t>(b);
+}
+
+/* Read(bits) for 0 <= bits <= 32 (entropy_codec.cc:186-194). */
+__device__ __forceinline__ uint32_t br_get(BitReader& br, uint32_t b) {
+    if (b == 0) return 0;
+    if (br.nb < static_cast<int32_t>(b)) br_refill(br);
+    const uint32_t v = static_cast<uint32_t>(br.win >> (64 - b));
+    br_skip(br, b);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t leading_ones(uint64_t w) {
+    const uint64_t inv = ~w;
+    return inv ? static_cast<uint32_t>(__builtin_clzll(inv)) : 64u;
+}
+
+/* GRRead (entropy_codec.cc:196-205): unary prefix counted with one
+ * leading-ones scan; 32 ones is the escape to a raw 32-bit value. */
+__device__ __forceinline__ uint64_t br_gr(BitReader& br, uint32_t kR) {
+    br_refill(br);
+    const uint32_t ones = leading_ones(br.win);
+    if (ones >= 32) {
+        br_skip(br, 32);
+        return br_get(br, 32);
+    }
+    br_skip(br, ones + 1);
+    return (static_cast<uint64_t>(ones) << kR) | br_get(br, kR);
+}
+
+/* k_RP adaptation (entropy_codec.cc:240-248, 259-267). */
+__device__ __forceinline__ uint32_t adapt_krp(uint32_t kRP, uint64_t p) {
+    if (p) {
+        const uint32_t pp = p > 200 ? 200u : static_cast<uint32_t>(p);
+        const uint32_t v = kRP + pp - 1;
+        return v > 128 ? 128u : v;
+    }
+    return kRP < 2 ? 0u : kRP - 2;
+}
+
+/* Unsign2Sign (entropy_codec.h:73-81) for u < 2^32. */
+__device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
+    const uint32_t h = static_cast<uint32_t>(u >> 1);
+    return static_cast<int32_t>(h ^ (0u - static_cast<uint32_t>(u & 1u)));
+}
+
+/* Decodes 3n symbols into coef[0, 3n) (planar Y, U, V) dequantised with qp.
+ * Called by all 64 lanes of one wavefront; control flow is wave-uniform. */
+__device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src, uint32_t nbytes, uint32_t total, uint32_t qp,
+                                float* coef) {
+    BitReader br;
+    br_init(br, src, nbytes);
+    const uint32_t lane = lane_id();
+    int32_t acc = 0;
+    uint32_t acc_base = 0;
+    uint32_t n = 0, kP = 0, kRP = 8;
+    int st = PVVC_OK;
+
+    auto store = [&](uint32_t base, int32_t v) {
+        const uint32_t i = base + lane;
+        if (i < total) coef[i] = static_cast<float>(static_cast<long long>(v) * static_cast<long long>(qp));
+    };
+    auto emit = [&](int32_t s) {
+        if (n - acc_base >= 64u) {
+            store(acc_base, acc);
+            acc = 0;
+            const uint32_t nbase = n & ~63u;
+            for (uint32_t b = acc_base + 64u; b < nbase; b += 64u) store(b, 0);
+            acc_base = nbase;
+        }
+        acc = (lane == n - acc_base) ? s : acc;
+        ++n;
+    };
+
+    while (n < total) {
+        uint32_t k = kP >> 2;
+        const uint32_t kR = kRP >> 2;
+        if (k == 0) {
+            /* no run-length coding (:255-276) */
+            const uint64_t u = br_gr(br, kR);
+            if (u >= (1ull << 32)) {
+                st = PVVC_ERR_SYMBOL_RANGE;
+                break;
+            }
+            emit(unsign2sign(u));
+            kRP = adapt_krp(kRP, u >> kR);
+            kP = u ? (kP ? kP - 1 : 0u) : kP + 3;
+        } else {
+            /* run-length coding (:221-252) */
+            uint64_t m = 0;
+            bool bad = false;
+            for (;;) {
+                br_refill(br);
+                const uint32_t t = leading_ones(br.win);
+                for (uint32_t i = 0; i < t; ++i) {
+                    m += 1ull << k;
+                    kP += 2;
+                    k = kP >> 2;
+                    if (k > 30) {
+                        bad = true;
+                        break;
+                    }
+                }
+                if (bad) break;
+                if (static_cast<int32_t>(t) < br.nb) {
+                    br_skip(br, t + 1);
+                    break;
+                }
+                br_skip(br, t);
+            }
+            if (bad) {
+                st = PVVC_ERR_BITSTREAM;
+                break;
+            }
+            m += br_get(br, k);
+            if (m >= static_cast<uint64_t>(total - n)) {
+                n = total;
+                break;
+            }
+            n += static_cast<uint32_t>(m);
+            const uint64_t u = br_gr(br, kR);
+            if (u + 1 >= (1ull << 32)) {
+                st = PVVC_ERR_SYMBOL_RANGE;
+                break;
+            }
+            emit(unsign2sign(u + 1));
+            kRP = adapt_krp(kRP, u >> kR);
+            kP = kP ? kP - 1 : 0u;
+        }
+    }
+    store(acc_base, acc);
+    for (uint32_t b = acc_base + 64u; b < total; b += 64u) store(b, 0);
+    const uint64_t consumed = static_cast<uint64_t>(br.wpos) * 32u - static_cast<uint64_t>(br.nb);
+    if (st == PVVC_OK && consumed > static_cast<uint64_t>(nbytes) * 8u) st = PVVC_ERR_BITSTREAM;
+    return st;
+}
+
+/* ======================================================================== */
+/* Device: octree structure (invert_raht_octree.cc:122-254)                  */
+/* ======================================================================== */
+
+__device__ __forceinline__ uint32_t prefix_lt(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+/* Exclusive wave scan of v in [0, 15] by bit planes; *total gets the sum. */
+__device__ __forceinline__ uint32_t wave_scan_small(uint32_t v, uint32_t* total) {
+    uint32_t r = 0, t = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint64_t m = __ballot((v >> b) & 1u);
+        r += prefix_lt(m) << b;
+        t += static_cast<uint32_t>(__popcll(m)) << b;
+    }
+    *total = t;
+    return r;
+}
+
+/* Pass 1, one wavefront: level sizes and first-child indices. */
+__device__ __forceinline__ int tree_structure(const uint8_t* __restrict__ geo, uint32_t geo_size, uint32_t n, uint32_t* fc,
+                              LevelMeta& meta) {
+    const uint32_t lane = lane_id();
+    const uint32_t h = geo[24];
+    if (h == 0 || h > static_cast<uint32_t>(kMaxHeight)) return PVVC_ERR_BAD_SLICE;
+    uint32_t base = 0, cnt = 1, wmax = 0;
+    if (lane == 0) {
+        meta.cnt[0] = 1;
+        meta.base[0] = 0;
+    }
+    for (uint32_t l = 0; l + 1 < h; ++l) {
+        if (25ull + base + cnt > geo_size) return PVVC_ERR_BAD_SLICE;
+        uint32_t next = 0;
+        for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            const uint32_t byte = j < cnt ? geo[25 + base + j] : 1u;
+            /* an internal node without children is never written by the
+             * encoder (raht_octree.cc:154-167); rejecting it keeps every
+             * level within n nodes and the AC positions closed-form */
+            if (__ballot(byte == 0u)) return PVVC_ERR_BAD_SLICE;
+            uint32_t tot;
+            const uint32_t ex = wave_scan_small(j < cnt ? __popc(byte) : 0u, &tot);
+            if (j < cnt) fc[base + j] = next + ex;
+            next += tot;
+        }
+        base += cnt;
+        cnt = next;
+        if (lane == 0) {
+            meta.cnt[l + 1] = cnt;
+            meta.base[l + 1] = base;
+        }
+        if (l + 2 < h) wmax = cnt > wmax ? cnt : wmax;
+    }
+    if (lane == 0) {
+        meta.height = h;
+        meta.internal = base;
+        meta.wmax = wmax;
+    }
+    if (cnt != n) return PVVC_ERR_UNMATCHED_SIZE; /* invert_raht_octree.cc:188-190 */
+    return PVVC_OK;
+}
+
+/* Pass 2, one wavefront: subtree leaf counts (the node weights). */
+__device__ __forceinline__ void tree_weights(const uint8_t* __restrict__ geo, const uint32_t* fc, uint32_t* w, const LevelMeta& meta) {
+    const uint32_t lane = lane_id();
+    const uint32_t h = meta.height;
+    for (int l = static_cast<int>(h) - 2; l >= 0; --l) {
+        const uint32_t cnt = meta.cnt[l], base = meta.base[l], cbase = meta.base[l + 1];
+        const bool last = (l == static_cast<int>(h) - 2);
+        for (uint32_t j = lane; j < cnt; j += 64) {
+            const uint32_t pc = __popc(static_cast<uint32_t>(geo[25 + base + j]));
+            uint32_t s = pc;
+            if (!last) {
+                const uint32_t f = fc[base + j];
+                s = 0;
+                for (uint32_t r = 0; r < pc; ++r) s += w[cbase + f + r];
+            }
+            w[base + j] = s;
+        }
+        __threadfence_block();
+    }
+}
+
+/* ======================================================================== */
+/* Device: inverse RAHT + leaf output                                        */
+/* ======================================================================== */
+
+__device__ __forceinline__ uint8_t to_u8(float f) {
+    /* ConvertRGB (common.h:154-158) with saturation; NaN -> 0 */
+    float q = roundf(f);
+    if (!(q == q)) return 0;
+    q = q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q);
+    return static_cast<uint8_t>(q);
+}
+
+__device__ __forceinline__ void emit_leaf(uint32_t c, float cx, float cy, float cz, float Y, float U, float V,
+                                          const SliceArgs& a, float* ref, uint32_t ref_stride, bool predict,
+                                          const Sink& sink) {
+    if (predict) { /* invert_raht_octree.cc:304-308 */
+        Y = Y + ref[c];
+        U = U + ref[ref_stride + c];
+        V = V + ref[2 * ref_stride + c];
+    } else { /* :309-311 */
+        ref[c] = Y;
+        ref[ref_stride + c] = U;
+        ref[2 * ref_stride + c] = V;
+    }
+    const float* m = a.mv;
+    const float x = cx * m[0] + cy * m[1] + cz * m[2] + m[3];
+    const float y = cx * m[4] + cy * m[5] + cz * m[6] + m[7];
+    const float z = cx * m[8] + cy * m[9] + cz * m[10] + m[11];
+    const float r = Y + 1.4020f * (V - 128.0f);
+    const float g = Y - 0.3441f * (U - 128.0f) - 0.7141f * (V - 128.0f);
+    const float b = Y + 1.7720f * (U - 128.0f);
+    const uint64_t o = a.out_off + c;
+    if (sink.mode == PVVC_OUT_NATIVE) {
+        float* px = sink.xyz + 3 * o;
+        px[0] = x;
+        px[1] = y;
+        px[2] = z;
+        uint8_t* pc = sink.rgb + 3 * o;
+        pc[0] = to_u8(r);
+        pc[1] = to_u8(g);
+        pc[2] = to_u8(b);
+    } else {
+        Points p;
+        p.x = x;
+        p.y = y;
+        p.z = z;
+        if (sink.white_small && a.n < 10) {
+            p.r = p.g = p.b = 1.0f;
+        } else {
+            p.r = r / 255.0f;
+            p.g = g / 255.0f;
+            p.b = b / 255.0f;
+        }
+        sink.pts[o] = p;
+    }
+}
+
+/* One inverse Haar butterfly pair (octree_base.cc:186-206) on channel values;
+ * only called with w0, w1 > 0. */
+__device__ __forceinline__ void inv_haar(uint32_t w0, uint32_t w1, float g, float h, float& o0, float& o1) {
+    const float base = sqrtf(static_cast<float>(w0 + w1));
+    const float ta = sqrtf(static_cast<float>(w0)) / base;
+    const float tc = sqrtf(static_cast<float>(w1)) / base;
+    o0 = g * ta + h * (-tc);
+    o1 = g * tc + h * ta;
+}
+
+/* ======================================================================== */
+/* Device: one slice, one workgroup                                          */
+/* ======================================================================== */
+
+struct Arena {
+    uint8_t* lds;
+    uint32_t lds_bytes;
+    uint32_t lds_off;
+    uint8_t* gws;
+    uint64_t gws_bytes;
+    uint64_t g_off;
+    __device__ void* take(uint64_t bytes) {
+        bytes = (bytes + 15u) & ~uint64_t(15);
+        if (lds_off + bytes <= lds_bytes) {
+            void* p = lds + lds_off;
+            lds_off += static_cast<uint32_t>(bytes);
+            return p;
+        }
+        if (g_off + bytes <= gws_bytes) {
+            void* p = gws + g_off;
+            g_off += bytes;
+            return p;
+        }
+        return nullptr;
+    }
+};
+
+template <int WAVES>
+__device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar, const Sink& sink, LevelMeta& meta,
+                            uint32_t launch_id, bool allow_wait) {
+    constexpr uint32_t NT = 64u * WAVES;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const bool predict = (a.type & PVVC_TYPE_PREDICT) != 0;
+    const bool wait = predict && allow_wait && a.wait;
+    int st = PVVC_OK;
+
+    /* ---- validation (SetSlice, invert_raht_octree.cc:25-40) ---- */
+    if (!(a.type & PVVC_TYPE_VALID)) {
+        st = PVVC_ERR_BAD_SLICE;
+    } else if (!predict) {
+        if (!a.geo || a.geo_size < 25) st = PVVC_ERR_BAD_SLICE;
+        else if (a.n > slot->n_cap || a.geo_size > slot->geo_cap) st = PVVC_ERR_CAPACITY;
+    } else if (!wait) {
+        const uint32_t sn = slot->n;
+        if (sn == 0) st = PVVC_ERR_EMPTY_REFERENCE;
+        else if (sn != a.n) st = PVVC_ERR_UNMATCHED_SIZE;
+        else if (!a.geo || a.geo_size == 0) {
+            a.geo = slot->geo;
+            a.geo_size = slot->geo_size;
+        }
+    } else if (a.n > slot->n_cap || !a.geo || a.geo_size < 25) {
+        st = PVVC_ERR_CAPACITY;
+    }
+    if (st == PVVC_OK && a.n == 0) st = PVVC_ERR_UNMATCHED_SIZE;
+
+    float* coef = nullptr;
+    uint32_t* fc = nullptr;
+    uint32_t* wt = nullptr;
+    if (st == PVVC_OK) {
+        const uint64_t imax = a.geo_size - 25u;
+        coef = static_cast<float*>(ar.take(12ull * a.n));
+        fc = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
+        wt = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
+        if (!coef || !fc || !wt) st = PVVC_ERR_CAPACITY;
+    }
+
+    /* ---- R (RLGR) and T (tree) ---- */
+    if (st == PVVC_OK) {
+        if (WAVES == 1) {
+            int ts = tree_structure(a.geo, a.geo_size, a.n, fc, meta);
+            __syncthreads();
+            if (ts == PVVC_OK) tree_weights(a.geo, fc, wt, meta);
+            int rs = PVVC_OK;
+            if (ts == PVVC_OK) rs = rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
+            st = ts != PVVC_OK ? ts : rs;
+        } else {
+            if (wave == 0) {
+                int rs = rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
+                if (lane_id() == 0) meta.rlgr_status = rs;
+            } else if (wave == 1) {
+                int ts = tree_structure(a.geo, a.geo_size, a.n, fc, meta);
+                __threadfence_block();
+                if (ts == PVVC_OK) tree_weights(a.geo, fc, wt, meta);
+                if (lane_id() == 0) meta.tree_status = ts;
+            }
+            __syncthreads();
+            st = meta.tree_status != PVVC_OK ? meta.tree_status : meta.rlgr_status;
+        }
+    }
+    __syncthreads();
+
+    /* ---- wait for the intra slice of this patch in the same launch ---- */
+    if (st == PVVC_OK && wait) {
+        __shared__ int wait_ok;
+        if (tid == 0) {
+            uint32_t spins = 0;
+            int ok = 1;
+            while (__hip_atomic_load(&slot->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != launch_id) {
+                if (++spins > kSpinLimit) {
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            wait_ok = ok;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (!wait_ok) {
+            st = PVVC_ERR_TIMEOUT;
+        } else {
+            const uint32_t sn = __hip_atomic_load(&slot->n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (sn == 0) st = PVVC_ERR_EMPTY_REFERENCE;
+            else if (sn != a.n) st = PVVC_ERR_UNMATCHED_SIZE;
+        }
+    }
+
+    /* ---- H + M: top-down inverse RAHT, centres, leaf output ---- */
+    const uint32_t h = (st == PVVC_OK) ? meta.height : 0u;
+    float* pp = nullptr;
+    if (st == PVVC_OK && h > 1) {
+        const uint32_t wmax = meta.wmax > 0 ? meta.wmax : 1u;
+        pp = static_cast<float*>(ar.take(48ull * wmax));
+        if (!pp) st = PVVC_ERR_CAPACITY;
+    }
+    if (st == PVVC_OK) {
+        const uint32_t n = a.n;
+        float* ref = slot->ref;
+        const uint32_t rs = slot->n_cap;
+        /* root centre / range: LoadTreeCore (octree_base.cc:33-48) */
+        float hdr[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const uint32_t wv = (static_cast<uint32_t>(a.geo[4 * i]) << 24) | (static_cast<uint32_t>(a.geo[4 * i + 1]) << 16) |
+                                (static_cast<uint32_t>(a.geo[4 * i + 2]) << 8) | static_cast<uint32_t>(a.geo[4 * i + 3]);
+            hdr[i] = __uint_as_float(wv);
+        }
+        if (h == 1) {
+            if (tid == 0) emit_leaf(0, hdr[0], hdr[1], hdr[2], coef[n - 1], coef[2 * n - 1], coef[3 * n - 1], a, ref, rs,
+                                    predict, sink);
+        } else {
+            const uint32_t wmax = meta.wmax > 0 ? meta.wmax : 1u;
+            float* bufA = pp;
+            float* bufB = pp + 6 * wmax;
+            float rx = hdr[3], ry = hdr[4], rz = hdr[5];
+            for (uint32_t l = 0; l + 1 < h; ++l) {
+                const uint32_t cnt = meta.cnt[l], base = meta.base[l];
+                const uint32_t cbase = meta.base[l + 1];
+                const bool leaf_level = (l + 2 == h);
+                /* SubSpaceCenter: child range = range / 2, offset = child range / 2 */
+                const float sx = rx / 2.0f, sy = ry / 2.0f, sz = rz / 2.0f;
+                const float ox = sx / 2.0f, oy = sy / 2.0f, oz = sz / 2.0f;
+                float* cur = (l & 1) ? bufB : bufA;
+                float* nxt = (l & 1) ? bufA : bufB;
+                for (uint32_t j = tid; j < cnt; j += NT) {
+                    const uint32_t occ = a.geo[25 + base + j];
+                    const uint32_t f = fc[base + j];
+                    float dY, dU, dV, cx, cy, cz;
+                    if (l == 0) {
+                        dY = coef[n - 1];
+                        dU = coef[2 * n - 1];
+                        dV = coef[3 * n - 1];
+                        cx = hdr[0];
+                        cy = hdr[1];
+                        cz = hdr[2];
+                    } else {
+                        dY = cur[j];
+                        dU = cur[wmax + j];
+                        dV = cur[2 * wmax + j];
+                        cx = cur[3 * wmax + j];
+                        cy = cur[4 * wmax + j];
+                        cz = cur[5 * wmax + j];
+                    }
+                    /* node weights (octree.h:99 layout) */
+                    uint32_t w[16];
+                    uint32_t rank = 0;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const bool has = (occ >> (7 - i)) & 1u;
+                        w[8 + i] = has ? (leaf_level ? 1u : wt[cbase + f + rank]) : 0u;
+                        rank += has ? 1u : 0u;
+                    }
+                    w[4] = w[8] + w[12];
+                    w[5] = w[9] + w[13];
+                    w[6] = w[10] + w[14];
+                    w[7] = w[11] + w[15];
+                    w[2] = w[4] + w[6];
+                    w[3] = w[5] + w[7];
+                    /* InvertHierarchicalTransform (octree_base.cc:84-100) */
+                    float cY[16], cU[16], cV[16];
+                    cY[1] = dY;
+                    cU[1] = dU;
+                    cV[1] = dV;
+                    uint32_t r = cnt + f - j; /* reversed stream index of this node's first AC */
+                    constexpr int NW[8][2] = {{0, 0}, {2, 3}, {4, 6}, {5, 7}, {8, 12}, {9, 13}, {10, 14}, {11, 15}};
+#pragma unroll
+                    for (int idx = 1; idx < 8; ++idx) {
+                        const int ia = NW[idx][0], ib = NW[idx][1];
+                        const uint32_t w0 = w[ia], w1 = w[ib];
+                        if (w0 != 0 && w1 != 0) {
+                            const float hY = coef[n - 1 - r], hU = coef[2 * n - 1 - r], hV = coef[3 * n - 1 - r];
+                            ++r;
+                            const float base_ = sqrtf(static_cast<float>(w0 + w1));
+                            const float ta = sqrtf(static_cast<float>(w0)) / base_;
+                            const float tc = sqrtf(static_cast<float>(w1)) / base_;
+                            cY[ia] = cY[idx] * ta + hY * (-tc);
+                            cY[ib] = cY[idx] * tc + hY * ta;
+                            cU[ia] = cU[idx] * ta + hU * (-tc);
+                            cU[ib] = cU[idx] * tc + hU * ta;
+                            cV[ia] = cV[idx] * ta + hV * (-tc);
+                            cV[ib] = cV[idx] * tc + hV * ta;
+                        } else {
+                            /* one side empty: the transform passes g through to
+                             * the occupied side (T = identity up to signed zeros) */
+                            cY[ia] = cY[idx];
+                            cU[ia] = cU[idx];
+                            cV[ia] = cV[idx];
+                            cY[ib] = cY[idx];
+                            cU[ib] = cU[idx];
+                            cV[ib] = cV[idx];
+                        }
+                    }
+                    /* children: g_DC, centre (SubSpaceCenter), write */
+                    uint32_t k = 0;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        if ((occ >> (7 - i)) & 1u) {
+                            const uint32_t c = f + k;
+                            ++k;
+                            const float ccx = (i & 4) ? cx - ox : cx + ox;
+                            const float ccy = (i & 2) ? cy - oy : cy + oy;
+                            const float ccz = (i & 1) ? cz - oz : cz + oz;
+                            if (leaf_level) {
+                                emit_leaf(c, ccx, ccy, ccz, cY[8 + i], cU[8 + i], cV[8 + i], a, ref, rs, predict, sink);
+                            } else {
+                                nxt[c] = cY[8 + i];
+                                nxt[wmax + c] = cU[8 + i];
+                                nxt[2 * wmax + c] = cV[8 + i];
+                                nxt[3 * wmax + c] = ccx;
+                                nxt[4 * wmax + c] = ccy;
+                                nxt[5 * wmax + c] = ccz;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                rx = sx;
+                ry = sy;
+                rz = sz;
+            }
+        }
+    }
+
+    /* ---- intra: keep geometry for the GoP, publish ---- */
+    __syncthreads();
+    if (!predict && (a.type & PVVC_TYPE_VALID)) {
+        if (st == PVVC_OK) {
+            for (uint32_t i = tid; i < a.geo_size; i += NT) slot->geo[i] = a.geo[i];
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            slot->n = st == PVVC_OK ? a.n : 0u;
+            slot->geo_size = st == PVVC_OK ? a.geo_size : 0u;
+            slot->status = st;
+            __hip_atomic_store(&slot->ready, launch_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (tid == 0) {
+        slot->status = st;
+    }
+    return st;
+}
+
+/* ======================================================================== */
+/* Kernels                                                                   */
+/* ======================================================================== */
+
+extern __shared__ __attribute__((aligned(16))) uint8_t g_smem[];
+
+/* Drop-in: one workgroup per slice of the reference's SoA device arrays
+ * (CUDA2OpenGL.cu:37-66 processes the same arrays, one thread per slice). */
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void dropin_kernel(Points* out, const int* __restrict__ inner_offset,
+                                                          const int* __restrict__ index, const uint8_t* __restrict__ type,
+                                                          float* const* __restrict__ mv, const uint32_t* __restrict__ size,
+                                                          const uint8_t* __restrict__ qp,
+                                                          uint8_t* const* __restrict__ geometry,
+                                                          const uint32_t* __restrict__ geometry_size,
+                                                          uint8_t* const* __restrict__ color,
+                                                          const uint32_t* __restrict__ color_size, SlotDev* slots,
+                                                          int n_slots, int patch_size, uint32_t lds_bytes,
+                                                          int white_small, int* err_flag) {
+    __shared__ LevelMeta meta;
+    const int s = blockIdx.x;
+    if (s >= patch_size) return;
+    const int pidx = index[s];
+    if (pidx < 0 || pidx >= n_slots) {
+        if (threadIdx.x == 0) atomicOr(err_flag, 1);
+        return;
+    }
+    SlotDev* slot = slots + pidx;
+    SliceArgs a;
+    a.geo = geometry[s];
+    a.geo_size = geometry_size[s];
+    a.col = color[s];
+    a.col_size = color_size[s];
+    a.n = size[s];
+    a.type = type[s];
+    a.qp = qp[s];
+    a.wait = 0;
+    a.out_off = static_cast<uint64_t>(static_cast<int64_t>(inner_offset[s]));
+    const float* m = mv[s];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) a.mv[i] = m[i];
+    Sink sink{nullptr, nullptr, out, PVVC_OUT_POINTS, white_small};
+    Arena ar{g_smem, lds_bytes, 0, slot->ws, slot->ws_cap, 0};
+    int st = decode_slice<WAVES>(a, slot, ar, sink, meta, 0u, false);
+    if (st != PVVC_OK && threadIdx.x == 0) atomicOr(err_flag, 2);
+}
+
+__device__ __forceinline__ SliceArgs item_args(const ItemDev* __restrict__ it) {
+    SliceArgs a;
+    a.geo = it->geo;
+    a.geo_size = it->geo_size;
+    a.col = it->col;
+    a.col_size = it->col_size;
+    a.n = it->n;
+    a.type = it->type;
+    a.qp = it->qp;
+    a.wait = it->wait;
+    a.out_off = it->out_off;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) a.mv[i] = it->mv[i];
+    return a;
+}
+
+/* Native per-frame: one workgroup per item of one frame, per-slot workspace. */
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void frame_kernel(const ItemDev* __restrict__ items, uint32_t first,
+                                                         uint32_t count, SlotDev* slots, Sink sink,
+                                                         uint32_t lds_bytes, int* __restrict__ status) {
+    __shared__ LevelMeta meta;
+    const uint32_t i = first + blockIdx.x;
+    if (blockIdx.x >= count) return;
+    const ItemDev* it = items + i;
+    SlotDev* slot = slots + it->patch;
+    Arena ar{g_smem, lds_bytes, 0, slot->ws, slot->ws_cap, 0};
+    int st = decode_slice<WAVES>(item_args(it), slot, ar, sink, meta, 0u, false);
+    if (threadIdx.x == 0) status[i] = st;
+}
+
+/* Batched persistent: workgroups pop items from a length-sorted queue. */
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* __restrict__ items,
+                                                              const uint32_t* __restrict__ order, uint32_t count,
+                                                              uint32_t* queue, SlotDev* slots, Sink sink,
+                                                              uint32_t lds_bytes, uint8_t* slab, uint64_t slab_bytes,
+                                                              uint32_t launch_id, int* __restrict__ status) {
+    __shared__ LevelMeta meta;
+    __shared__ uint32_t next;
+    uint8_t* ws = slab + static_cast<uint64_t>(blockIdx.x) * slab_bytes;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t q = next;
+        __syncthreads();
+        if (q >= count) break;
+        const uint32_t i = order[q];
+        const ItemDev* it = items + i;
+        SlotDev* slot = slots + it->patch;
+        Arena ar{g_smem, lds_bytes, 0, ws, slab_bytes, 0};
+        int st = decode_slice<WAVES>(item_args(it), slot, ar, sink, meta, launch_id, true);
+        if (threadIdx.x == 0) status[i] = st;
+        __syncthreads();
+    }
+}
+
+__global__ void reset_counter(uint32_t* q) { *q = 0; }
+
+}  // namespace pvvc
+
+/* ======================================================================== */
+/* Host                                                                      */
+/* ======================================================================== */
+using namespace pvvc;
+
+namespace {
+
+std::atomic<int> g_last_error{0};
+hipStream_t g_stream = nullptr;
+int g_white_small = 0;
+std::mutex g_mu;
+
+void set_error(int e) {
+    if (e) g_last_error.store(e);
+}
+
+bool hip_ok(hipError_t e) {
+    if (e != hipSuccess) {
+        set_error(PVVC_ERR_HIP);
+        std::fprintf(stderr, "pvvc: HIP error %s\n", hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
+struct DeviceInfo {
+    int cu = 0, lds_max = 0, warp = 64;
+    char name[256] = {0};
+    bool ok = false;
+};
+DeviceInfo& devinfo() {
+    static DeviceInfo d;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) {
+            d.cu = p.multiProcessorCount;
+            d.lds_max = static_cast<int>(p.sharedMemPerBlock);
+            d.warp = p.warpSize;
+            std::snprintf(d.name, sizeof(d.name), "%s", p.gcnArchName);
+            d.ok = true;
+        }
+    });
+    return d;
+}
+
+/* Launch-time records of the last decode call: event pairs per main launch. */
+struct Timing {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    void reset() { used = 0; }
+    std::pair<hipEvent_t, hipEvent_t> next() {
+        while (ev.size() < used + 2) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            ev.push_back(e);
+        }
+        auto r = std::make_pair(ev[used], ev[used + 1]);
+        used += 2;
+        return r;
+    }
+};
+Timing g_timing;
+
+struct TuningState {
+    int waves = 1;
+    int lds_bytes = 48 * 1024;
+    int wgs_per_cu = 8;
+    int queue_chunk = 1;
+};
+TuningState g_dropin_tuning;
+
+int clamp_lds(int want) {
+    int mx = devinfo().lds_max > 0 ? devinfo().lds_max : 65536;
+    int reserve = static_cast<int>(sizeof(LevelMeta)) + 64;
+    want = std::max(0, std::min(want, mx - reserve));
+    return want & ~15;
+}
+
+template <typename K>
+void allow_lds(K kernel, int bytes) {
+    hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+/* ---- state slots (pvvc_state) ---- */
+struct StateInfo {
+    SlotDev* slots = nullptr;  /* device */
+    uint8_t* arena = nullptr;  /* device */
+    int n = 0;
+    int max_points = 0;
+    uint32_t geo_cap = 0;
+    uint64_t ws_cap = 0;
+    uint64_t slot_stride = 0;
+    int* err = nullptr;        /* device error word */
+};
+std::map<const void*, StateInfo> g_states;
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+/* geometry capacity: 2 internal nodes per point + header; workspace: coef
+ * 12n + first-child / weight arrays over the geometry + two level buffers. */
+uint32_t geo_cap_for(int max_points) { return static_cast<uint32_t>(64 + 2ull * max_points + 25); }
+uint64_t ws_cap_for(int max_points) {
+    uint64_t n = static_cast<uint64_t>(max_points);
+    return align256(12 * n + 2 * (4 * (geo_cap_for(max_points) + 1ull) + 16) + 48 * n + 64);
+}
+
+bool state_create(int n_patches, int max_points, StateInfo& si) {
+    si.n = n_patches;
+    si.max_points = max_points;
+    si.geo_cap = geo_cap_for(max_points);
+    si.ws_cap = ws_cap_for(max_points);
+    si.slot_stride = align256(si.geo_cap) + align256(12ull * max_points) + si.ws_cap;
+    if (!hip_ok(hipMalloc(&si.slots, sizeof(SlotDev) * std::max(1, n_patches)))) return false;
+    if (!hip_ok(hipMalloc(&si.arena, si.slot_stride * std::max(1, n_patches)))) {
+        hipFree(si.slots);
+        return false;
+    }
+    if (!hip_ok(hipMalloc(&si.err, sizeof(int)))) return false;
+    hipMemset(si.err, 0, sizeof(int));
+    std::vector<SlotDev> h(std::max(1, n_patches));
+    for (int i = 0; i < n_patches; ++i) {
+        SlotDev& s = h[i];
+        std::memset(&s, 0, sizeof(s));
+        uint8_t* b = si.arena + si.slot_stride * i;
+        s.geo = b;
+        s.ref = reinterpret_cast<float*>(b + align256(si.geo_cap));
+        s.ws = b + align256(si.geo_cap) + align256(12ull * max_points);
+        s.ws_cap = si.ws_cap;
+        s.geo_cap = si.geo_cap;
+        s.n_cap = static_cast<uint32_t>(max_points);
+    }
+    if (!hip_ok(hipMemcpy(si.slots, h.data(), sizeof(SlotDev) * n_patches, hipMemcpyHostToDevice))) return false;
+    return true;
+}
+
+void state_destroy(StateInfo& si) {
+    if (si.slots) hipFree(si.slots);
+    if (si.arena) hipFree(si.arena);
+    if (si.err) hipFree(si.err);
+    si = StateInfo{};
+}
+
+bool state_reset(StateInfo& si) {
+    std::vector<SlotDev> h(si.n);
+    if (!hip_ok(hipMemcpy(h.data(), si.slots, sizeof(SlotDev) * si.n, hipMemcpyDeviceToHost))) return false;
+    for (auto& s : h) {
+        s.n = 0;
+        s.geo_size = 0;
+        s.status = 0;
+        s.ready = 0;
+    }
+    return hip_ok(hipMemcpy(si.slots, h.data(), sizeof(SlotDev) * si.n, hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+/* ---- native decoder / batch ---- */
+struct pvvc_decoder {
+    StateInfo st;
+    pvvc_tuning tuning{1, 48 * 1024, 8, 1};
+    uint32_t launch_id = 0;
+    uint8_t* slab = nullptr;
+    uint64_t slab_bytes = 0, slab_total = 0;
+    uint32_t* queue = nullptr;
+};
+
+struct Segment {
+    uint32_t begin, count; /* range in order[] */
+    uint64_t ws_need;      /* per-item workspace bound */
+};
+
+struct pvvc_batch {
+    pvvc_decoder* dec = nullptr;
+    int n_frames = 0;
+    std::vector<int64_t> frame_off, frame_pts;
+    std::vector<uint32_t> frame_first, frame_count; /* items in slice order */
+    std::vector<Segment> segs;
+    uint32_t n_items = 0;
+    int64_t total_points = 0;
+    int64_t input_bytes = 0;
+    uint8_t* dev = nullptr;
+    ItemDev* items = nullptr;
+    uint32_t* order = nullptr;
+    int* status = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_output = -1;
+    void* graph_ptrs[3] = {nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+template <int W>
+void launch_frame(const pvvc_batch* b, int f, const Sink& sink, uint32_t lds, hipStream_t s) {
+    uint32_t cnt = b->frame_count[f];
+    if (!cnt) return;
+    hipLaunchKernelGGL(frame_kernel<W>, dim3(cnt), dim3(64 * W), lds, s, b->items, b->frame_first[f], cnt,
+                       b->dec->st.slots, sink, lds, b->status);
+}
+
+template <int W>
+void launch_persistent(pvvc_batch* b, const Segment& sg, const Sink& sink, uint32_t lds, uint32_t grid,
+                       uint32_t launch_id, hipStream_t s) {
+    pvvc_decoder* d = b->dec;
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, d->queue);
+    hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b->items, b->order + sg.begin, sg.count,
+                       d->queue, d->st.slots, sink, lds, d->slab, d->slab_bytes, launch_id, b->status);
+}
+
+template <int W>
+void set_lds_attrs(int bytes) {
+    allow_lds(frame_kernel<W>, bytes);
+    allow_lds(persistent_kernel<W>, bytes);
+    allow_lds(dropin_kernel<W>, bytes);
+}
+
+void set_all_lds_attrs(int bytes) {
+    set_lds_attrs<1>(bytes);
+    set_lds_attrs<2>(bytes);
+    set_lds_attrs<4>(bytes);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pvvc_last_error(void) { return g_last_error.exchange(0); }
+
+int pvvc_set_stream(void* stream) {
+    g_stream = static_cast<hipStream_t>(stream);
+    return PVVC_OK;
+}
+
+int pvvc_set_white_small_patches(int enable) {
+    g_white_small = enable ? 1 : 0;
+    return PVVC_OK;
+}
+
+int pvvc_device_info(int* cu_count, int* lds_per_block, int* warp_size, char* name, int name_len) {
+    DeviceInfo& d = devinfo();
+    if (!d.ok) return PVVC_ERR_HIP;
+    if (cu_count) *cu_count = d.cu;
+    if (lds_per_block) *lds_per_block = d.lds_max;
+    if (warp_size) *warp_size = d.warp;
+    if (name && name_len > 0) std::snprintf(name, name_len, "%s", d.name);
+    return PVVC_OK;
+}
+
+void* pvvc_dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (!hip_ok(hipMalloc(&p, bytes ? bytes : 1))) return nullptr;
+    return p;
+}
+void pvvc_dev_free(void* p) {
+    if (p) hipFree(p);
+}
+int pvvc_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)) ? PVVC_OK : PVVC_ERR_HIP;
+}
+int pvvc_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)) ? PVVC_OK : PVVC_ERR_HIP;
+}
+int pvvc_memset(void* dst, int value, size_t bytes) {
+    return hip_ok(hipMemset(dst, value, bytes)) ? PVVC_OK : PVVC_ERR_HIP;
+}
+int pvvc_synchronize(void) { return hip_ok(hipStreamSynchronize(g_stream)) ? PVVC_OK : PVVC_ERR_HIP; }
+
+/* Milliseconds of the main kernel launches of the last decode call. */
+double pvvc_last_kernel_ms(void) {
+    double tot = 0;
+    for (size_t i = 0; i + 1 < g_timing.used; i += 2) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, g_timing.ev[i], g_timing.ev[i + 1]) == hipSuccess) tot += ms;
+    }
+    return tot;
+}
+int pvvc_last_launch_times(double* ms, int max) {
+    int k = 0;
+    for (size_t i = 0; i + 1 < g_timing.used && k < max; i += 2, ++k) {
+        float t = 0;
+        hipEventElapsedTime(&t, g_timing.ev[i], g_timing.ev[i + 1]);
+        if (ms) ms[k] = t;
+    }
+    return static_cast<int>(g_timing.used / 2);
+}
+
+/* ---- pvvc_state (drop-in) ---- */
+pvvc_state* pvvc_state_alloc(int n_patches, int max_points) {
+    if (n_patches <= 0 || max_points <= 0) {
+        set_error(PVVC_ERR_INVALID_ARG);
+        return nullptr;
+    }
+    StateInfo si;
+    if (!state_create(n_patches, max_points, si)) {
+        state_destroy(si);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_states[si.slots] = si;
+    return reinterpret_cast<pvvc_state*>(si.slots);
+}
+
+void pvvc_state_free(pvvc_state* state) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_states.find(state);
+    if (it == g_states.end()) return;
+    state_destroy(it->second);
+    g_states.erase(it);
+}
+
+int pvvc_state_patch_status(const pvvc_state* state, int patch) {
+    StateInfo si;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_states.find(state);
+        if (it == g_states.end()) return PVVC_ERR_INVALID_ARG;
+        si = it->second;
+    }
+    if (patch < 0 || patch >= si.n) return PVVC_ERR_INVALID_ARG;
+    if (!hip_ok(hipStreamSynchronize(g_stream))) return PVVC_ERR_HIP;
+    SlotDev s;
+    if (!hip_ok(hipMemcpy(&s, si.slots + patch, sizeof(s), hipMemcpyDeviceToHost))) return PVVC_ERR_HIP;
+    return s.status;
+}
+
+int pvvc_set_dropin_tuning(int waves, int lds_bytes) {
+    if (waves != 1 && waves != 2 && waves != 4) return PVVC_ERR_INVALID_ARG;
+    g_dropin_tuning.waves = waves;
+    g_dropin_tuning.lds_bytes = lds_bytes;
+    return PVVC_OK;
+}
+
+void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, int* inner_offset, int* index,
+                        uint8_t* type, float** mv, uint32_t* size, uint8_t* qp, uint8_t** geometry,
+                        uint32_t* geometry_size, uint8_t** color, uint32_t* color_size, pvvc_state* state,
+                        int patch_size) {
+    (void)grid;
+    (void)block;
+    (void)timestamp;
+    if (patch_size <= 0) return;
+    StateInfo si;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_states.find(state);
+        if (it == g_states.end()) {
+            set_error(PVVC_ERR_INVALID_ARG);
+            return;
+        }
+        si = it->second;
+    }
+    const int lds = clamp_lds(g_dropin_tuning.lds_bytes);
+    static std::once_flag once;
+    std::call_once(once, [] { set_all_lds_attrs(clamp_lds(1 << 30)); });
+    g_timing.reset();
+    auto ev = g_timing.next();
+    hipEventRecord(ev.first, g_stream);
+    switch (g_dropin_tuning.waves) {
+        case 2:
+            hipLaunchKernelGGL(dropin_kernel<2>, dim3(patch_size), dim3(128), lds, g_stream, cudaData, inner_offset,
+                               index, type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
+                               patch_size, static_cast<uint32_t>(lds), g_white_small, si.err);
+            break;
+        case 4:
+            hipLaunchKernelGGL(dropin_kernel<4>, dim3(patch_size), dim3(256), lds, g_stream, cudaData, inner_offset,
+                               index, type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
+                               patch_size, static_cast<uint32_t>(lds), g_white_small, si.err);
+            break;
+        default:
+            hipLaunchKernelGGL(dropin_kernel<1>, dim3(patch_size), dim3(64), lds, g_stream, cudaData, inner_offset,
+                               index, type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
+                               patch_size, static_cast<uint32_t>(lds), g_white_small, si.err);
+    }
+    hip_ok(hipGetLastError());
+    hipEventRecord(ev.second, g_stream);
+}
+
+/* ---- drop-in staging (test_decode_only.cc:97-123 in one copy) ---- */
+int pvvc_dropin_stage(const pvvc_frame* f, pvvc_dropin_frame* d) {
+    if (!f || !d) return PVVC_ERR_INVALID_ARG;
+    std::memset(d, 0, sizeof(*d));
+    const uint32_t S = f->slice_cnt;
+    /* layout: [ptr arrays][u32/int arrays][u8 arrays][mv][blobs] */
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t o_mvp = 0, o_geop = al(o_mvp + 8 * S), o_colp = al(o_geop + 8 * S);
+    size_t o_off = al(o_colp + 8 * S), o_idx = al(o_off + 4 * S), o_size = al(o_idx + 4 * S);
+    size_t o_gsz = al(o_size + 4 * S), o_csz = al(o_gsz + 4 * S), o_type = al(o_csz + 4 * S);
+    size_t o_qp = al(o_type + S), o_mv = al(o_qp + S), o_data = al(o_mv + 64 * S);
+    size_t total = o_data + f->data.size();
+    uint8_t* dev = static_cast<uint8_t*>(pvvc_dev_alloc(total));
+    if (!dev) return PVVC_ERR_HIP;
+    std::vector<uint8_t> h(total, 0);
+    std::vector<int> off(S);
+    int64_t acc = 0;
+    for (uint32_t i = 0; i < S; ++i) {
+        off[i] = static_cast<int>(acc);
+        acc += f->size[i];
+        uint64_t mvp = reinterpret_cast<uint64_t>(dev + o_mv + 64 * i);
+        uint64_t gp = f->geo_size[i] ? reinterpret_cast<uint64_t>(dev + o_data + f->geo_off[i]) : 0;
+        uint64_t cp = f->col_size[i] ? reinterpret_cast<uint64_t>(dev + o_data + f->col_off[i]) : 0;
+        std::memcpy(&h[o_mvp + 8 * i], &mvp, 8);
+        std::memcpy(&h[o_geop + 8 * i], &gp, 8);
+        std::memcpy(&h[o_colp + 8 * i], &cp, 8);
+    }
+    std::memcpy(&h[o_off], off.data(), 4 * S);
+    std::memcpy(&h[o_idx], f->index.data(), 4 * S);
+    std::memcpy(&h[o_size], f->size.data(), 4 * S);
+    std::memcpy(&h[o_gsz], f->geo_size.data(), 4 * S);
+    std::memcpy(&h[o_csz], f->col_size.data(), 4 * S);
+    std::memcpy(&h[o_type], f->type.data(), S);
+    std::memcpy(&h[o_qp], f->qp.data(), S);
+    std::memcpy(&h[o_mv], f->mv.data(), 64 * S);
+    std::memcpy(&h[o_data], f->data.data(), f->data.size());
+    if (pvvc_memcpy_h2d(dev, h.data(), total) != PVVC_OK) {
+        pvvc_dev_free(dev);
+        return PVVC_ERR_HIP;
+    }
+    d->mv = reinterpret_cast<float**>(dev + o_mvp);
+    d->geometry = reinterpret_cast<uint8_t**>(dev + o_geop);
+    d->color = reinterpret_cast<uint8_t**>(dev + o_colp);
+    d->inner_offset = reinterpret_cast<int*>(dev + o_off);
+    d->index = reinterpret_cast<int*>(dev + o_idx);
+    d->size = reinterpret_cast<uint32_t*>(dev + o_size);
+    d->geometry_size = reinterpret_cast<uint32_t*>(dev + o_gsz);
+    d->color_size = reinterpret_cast<uint32_t*>(dev + o_csz);
+    d->type = dev + o_type;
+    d->qp = dev + o_qp;
+    d->slice_cnt = static_cast<int>(S);
+    d->point_cnt = acc;
+    d->blob = dev;
+    d->blob_bytes = total;
+    return PVVC_OK;
+}
+
+void pvvc_dropin_release(pvvc_dropin_frame* d) {
+    if (d && d->blob) pvvc_dev_free(d->blob);
+    if (d) std::memset(d, 0, sizeof(*d));
+}
+
+/* ---- native decoder ---- */
+int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_patch) {
+    if (!out || max_patches <= 0 || max_points_per_patch <= 0) return PVVC_ERR_INVALID_ARG;
+    auto* d = new pvvc_decoder();
+    if (!state_create(max_patches, max_points_per_patch, d->st)) {
+        state_destroy(d->st);
+        delete d;
+        return PVVC_ERR_HIP;
+    }
+    if (!hip_ok(hipMalloc(&d->queue, 64))) {
+        state_destroy(d->st);
+        delete d;
+        return PVVC_ERR_HIP;
+    }
+    *out = d;
+    return PVVC_OK;
+}
+
+int pvvc_decoder_destroy(pvvc_decoder* d) {
+    if (!d) return PVVC_ERR_INVALID_ARG;
+    hipStreamSynchronize(g_stream);
+    state_destroy(d->st);
+    if (d->slab) hipFree(d->slab);
+    if (d->queue) hipFree(d->queue);
+    delete d;
+    return PVVC_OK;
+}
+
+int pvvc_decoder_set_tuning(pvvc_decoder* d, const pvvc_tuning* t) {
+    if (!d || !t) return PVVC_ERR_INVALID_ARG;
+    pvvc_tuning n = d->tuning;
+    if (t->waves_per_patch) {
+        if (t->waves_per_patch != 1 && t->waves_per_patch != 2 && t->waves_per_patch != 4) return PVVC_ERR_INVALID_ARG;
+        n.waves_per_patch = t->waves_per_patch;
+    }
+    if (t->lds_bytes) n.lds_bytes = t->lds_bytes;
+    if (t->wgs_per_cu) n.wgs_per_cu = t->wgs_per_cu;
+    if (t->queue_chunk) n.queue_chunk = t->queue_chunk;
+    d->tuning = n;
+    return PVVC_OK;
+}
+
+int pvvc_decoder_get_tuning(const pvvc_decoder* d, pvvc_tuning* t) {
+    if (!d || !t) return PVVC_ERR_INVALID_ARG;
+    *t = d->tuning;
+    return PVVC_OK;
+}
+
+int pvvc_decoder_reset(pvvc_decoder* d) {
+    if (!d) return PVVC_ERR_INVALID_ARG;
+    if (!hip_ok(hipStreamSynchronize(g_stream))) return PVVC_ERR_HIP;
+    return state_reset(d->st) ? PVVC_OK : PVVC_ERR_HIP;
+}
+
+int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_frames, pvvc_batch** out) {
+    if (!d || !frames || n_frames <= 0 || !out) return PVVC_ERR_INVALID_ARG;
+    auto* b = new pvvc_batch();
+    b->dec = d;
+    b->n_frames = n_frames;
+    /* host image: [items][order][status][frame blobs] */
+    uint32_t n_items = 0;
+    size_t data_bytes = 0;
+    for (int f = 0; f < n_frames; ++f) {
+        if (!frames[f]) {
+            delete b;
+            return PVVC_ERR_INVALID_ARG;
+        }
+        n_items += frames[f]->slice_cnt;
+        data_bytes += (frames[f]->data.size() + 255) & ~size_t(255);
+    }
+    b->n_items = n_items;
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_items = 0, o_order = al(sizeof(ItemDev) * n_items), o_status = al(o_order + 4ull * n_items);
+    const size_t o_data = al(o_status + 4ull * n_items);
+    const size_t total = o_data + data_bytes;
+    if (!hip_ok(hipMalloc(&b->dev, std::max<size_t>(total, 256)))) {
+        delete b;
+        return PVVC_ERR_HIP;
+    }
+    std::vector<uint8_t> h(total, 0);
+    std::vector<ItemDev> items(n_items);
+    std::vector<uint32_t> patch_of(n_items);
+    /* per patch: the item index of the intra slice seen in this batch, if any */
+    std::map<int, uint32_t> intra_item;
+    std::map<int, int> last_frame_of_patch;
+    size_t data_at = 0;
+    int64_t pts_acc = 0;
+    uint32_t it = 0;
+    bool split_needed = false;
+    std::vector<uint32_t> seg_starts{0};
+    for (int f = 0; f < n_frames; ++f) {
+        const pvvc_frame* fr = frames[f];
+        b->frame_first.push_back(it);
+        b->frame_count.push_back(fr->slice_cnt);
+        b->frame_off.push_back(pts_acc);
+        b->frame_pts.push_back(static_cast<int64_t>(fr->points));
+        uint8_t* dbase = b->dev + o_data + data_at;
+        std::memcpy(&h[o_data + data_at], fr->data.data(), fr->data.size());
+        /* A patch's new intra slice after earlier items of that patch in the
+         * same batch starts a new segment (launch), so that it never rewrites
+         * state that an earlier predictive item still reads. */
+        bool new_seg = false;
+        for (uint32_t s = 0; s < fr->slice_cnt; ++s) {
+            int p = fr->index[s];
+            if (!(fr->type[s] & PVVC_TYPE_PREDICT) && last_frame_of_patch.count(p)) new_seg = true;
+        }
+        if (new_seg && it != 0) {
+            seg_starts.push_back(it);
+            intra_item.clear();
+            last_frame_of_patch.clear();
+            split_needed = true;
+        }
+        int64_t inner = 0;
+        for (uint32_t s = 0; s < fr->slice_cnt; ++s, ++it) {
+            ItemDev& x = items[it];
+            std::memset(&x, 0, sizeof(x));
+            const int p = fr->index[s];
+            x.geo = fr->geo_size[s] ? dbase + fr->geo_off[s] : nullptr;
+            x.geo_size = fr->geo_size[s];
+            x.col = fr->col_size[s] ? dbase + fr->col_off[s] : nullptr;
+            x.col_size = fr->col_size[s];
+            x.n = fr->size[s];
+            x.patch = p;
+            x.type = fr->type[s];
+            x.qp = fr->qp[s];
+            x.out_off = static_cast<uint64_t>(pts_acc + inner);
+            for (int k = 0; k < 12; ++k) x.mv[k] = fr->mv[16 * s + k];
+            if (p < 0 || p >= d->st.n) {
+                pvvc_dev_free(b->dev);
+                delete b;
+                return PVVC_ERR_INVALID_ARG;
+            }
+            if (x.type & PVVC_TYPE_PREDICT) {
+                auto ii = intra_item.find(p);
+                if (ii != intra_item.end()) {
+                    const ItemDev& in = items[ii->second];
+                    x.geo = in.geo;
+                    x.geo_size = in.geo_size;
+                    x.wait = 1;
+                } else {
+                    x.geo = nullptr;
+                    x.geo_size = 0;
+                }
+            } else {
+                intra_item[p] = it;
+            }
+            last_frame_of_patch[p] = f;
+            patch_of[it] = p;
+            inner += fr->size[s];
+        }
+        pts_acc += inner;
+        data_at += (fr->data.size() + 255) & ~size_t(255);
+        b->input_bytes += 82ll * fr->slice_cnt;
+        for (uint32_t s = 0; s < fr->slice_cnt; ++s) b->input_bytes += fr->geo_size[s] + fr->col_size[s];
+    }
+    (void)split_needed;
+    b->total_points = pts_acc;
+    /* queue order per segment: longest first; intra before predictive of equal length */
+    std::vector<uint32_t> order(n_items);
+    seg_starts.push_back(n_items);
+    for (size_t sidx = 0; sidx + 1 < seg_starts.size(); ++sidx) {
+        uint32_t lo = seg_starts[sidx], hi = seg_starts[sidx + 1];
+        std::vector<uint32_t> ids(hi - lo);
+        std::iota(ids.begin(), ids.end(), lo);
+        /* key: a patch's intra item inherits the largest size of its patch */
+        std::map<int, uint32_t> patch_max;
+        for (uint32_t i : ids) patch_max[patch_of[i]] = std::max(patch_max[patch_of[i]], items[i].n);
+        std::stable_sort(ids.begin(), ids.end(), [&](uint32_t a, uint32_t c) {
+            const ItemDev &A = items[a], &C = items[c];
+            uint32_t ka = (A.type & PVVC_TYPE_PREDICT) ? A.n : patch_max[patch_of[a]];
+            uint32_t kc = (C.type & PVVC_TYPE_PREDICT) ? C.n : patch_max[patch_of[c]];
+            if (ka != kc) return ka > kc;
+            bool pa = A.type & PVVC_TYPE_PREDICT, pc = C.type & PVVC_TYPE_PREDICT;
+            if (pa != pc) return !pa;
+            return a < c;
+        });
+        uint64_t need = 0;
+        for (uint32_t i : ids) {
+            const ItemDev& x = items[i];
+            uint64_t g = x.geo_size ? x.geo_size : d->st.geo_cap;
+            uint64_t bound = 12ull * x.n + 2 * (4 * (g + 1) + 16) + 48ull * x.n + 64;
+            need = std::max(need, bound);
+        }
+        std::copy(ids.begin(), ids.end(), order.begin() + lo);
+        b->segs.push_back(Segment{lo, hi - lo, (need + 255) & ~uint64_t(255)});
+    }
+    std::memcpy(&h[o_items], items.data(), sizeof(ItemDev) * n_items);
+    std::memcpy(&h[o_order], order.data(), 4ull * n_items);
+    if (!hip_ok(hipMemcpy(b->dev, h.data(), total, hipMemcpyHostToDevice))) {
+        hipFree(b->dev);
+        delete b;
+        return PVVC_ERR_HIP;
+    }
+    b->items = reinterpret_cast<ItemDev*>(b->dev + o_items);
+    b->order = reinterpret_cast<uint32_t*>(b->dev + o_order);
+    b->status = reinterpret_cast<int*>(b->dev + o_status);
+    *out = b;
+    return PVVC_OK;
+}
+
+void pvvc_batch_free(pvvc_batch* b) {
+    if (!b) return;
+    hipStreamSynchronize(g_stream);
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->dev) hipFree(b->dev);
+    delete b;
+}
+
+int pvvc_batch_frames(const pvvc_batch* b) { return b ? b->n_frames : 0; }
+int64_t pvvc_batch_frame_points(const pvvc_batch* b, int f) {
+    return (b && f >= 0 && f < b->n_frames) ? b->frame_pts[f] : -1;
+}
+int64_t pvvc_batch_frame_offset(const pvvc_batch* b, int f) {
+    return (b && f >= 0 && f < b->n_frames) ? b->frame_off[f] : -1;
+}
+int64_t pvvc_batch_input_bytes(const pvvc_batch* b) { return b ? b->input_bytes : -1; }
+int pvvc_batch_items(const pvvc_batch* b) { return b ? static_cast<int>(b->n_items) : 0; }
+
+int pvvc_batch_status(pvvc_batch* b, int* status, int n) {
+    if (!b || !status) return PVVC_ERR_INVALID_ARG;
+    if (!hip_ok(hipStreamSynchronize(g_stream))) return PVVC_ERR_HIP;
+    int k = std::min<int>(n, static_cast<int>(b->n_items));
+    return hip_ok(hipMemcpy(status, b->status, 4ull * k, hipMemcpyDeviceToHost)) ? PVVC_OK : PVVC_ERR_HIP;
+}
+
+int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, float* xyz, uint8_t* rgb,
+                      Points* points) {
+    if (!d || !b || b->dec != d) return PVVC_ERR_INVALID_ARG;
+    if (output == PVVC_OUT_NATIVE && (!xyz || !rgb)) return PVVC_ERR_INVALID_ARG;
+    if (output == PVVC_OUT_POINTS && !points) return PVVC_ERR_INVALID_ARG;
+    static std::once_flag once;
+    std::call_once(once, [] { set_all_lds_attrs(clamp_lds(1 << 30)); });
+    Sink sink{xyz, rgb, points, output, g_white_small};
+    const int W = d->tuning.waves_per_patch;
+    const uint32_t lds = static_cast<uint32_t>(clamp_lds(d->tuning.lds_bytes));
+    g_timing.reset();
+    if (mode == PVVC_MODE_PER_FRAME || mode == PVVC_MODE_GRAPH) {
+        if (mode == PVVC_MODE_GRAPH) {
+            void* ptrs[3] = {xyz, rgb, points};
+            bool same = b->graph && b->graph_output == output && std::equal(ptrs, ptrs + 3, b->graph_ptrs);
+            if (!same) {
+                if (b->graph) hipGraphExecDestroy(b->graph);
+                b->graph = nullptr;
+                hipStream_t cs;
+                if (!hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking))) return PVVC_ERR_HIP;
+                hipGraph_t g;
+                if (!hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal))) return PVVC_ERR_HIP;
+                for (int f = 0; f < b->n_frames; ++f) {
+                    if (W == 2) launch_frame<2>(b, f, sink, lds, cs);
+                    else if (W == 4) launch_frame<4>(b, f, sink, lds, cs);
+                    else launch_frame<1>(b, f, sink, lds, cs);
+                }
+                if (!hip_ok(hipStreamEndCapture(cs, &g))) return PVVC_ERR_HIP;
+                if (!hip_ok(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0))) return PVVC_ERR_HIP;
+                hipGraphDestroy(g);
+                hipStreamDestroy(cs);
+                b->graph_output = output;
+                std::copy(ptrs, ptrs + 3, b->graph_ptrs);
+            }
+            auto ev = g_timing.next();
+            hipEventRecord(ev.first, g_stream);
+            hip_ok(hipGraphLaunch(b->graph, g_stream));
+            hipEventRecord(ev.second, g_stream);
+            return PVVC_OK;
+        }
+        for (int f = 0; f < b->n_frames; ++f) {
+            auto ev = g_timing.next();
+            hipEventRecord(ev.first, g_stream);
+            if (W == 2) launch_frame<2>(b, f, sink, lds, g_stream);
+            else if (W == 4) launch_frame<4>(b, f, sink, lds, g_stream);
+            else launch_frame<1>(b, f, sink, lds, g_stream);
+            hipEventRecord(ev.second, g_stream);
+        }
+        return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
+    }
+    if (mode != PVVC_MODE_PERSISTENT) return PVVC_ERR_INVALID_ARG;
+    DeviceInfo& di = devinfo();
+    const uint32_t cap_grid = static_cast<uint32_t>(std::max(1, di.cu) * std::max(1, d->tuning.wgs_per_cu));
+    uint64_t need = 0;
+    uint32_t grid_max = 0;
+    for (auto& sg : b->segs) {
+        need = std::max(need, sg.ws_need);
+        grid_max = std::max(grid_max, std::min(sg.count, cap_grid));
+    }
+    if (need * grid_max > d->slab_total || need > d->slab_bytes) {
+        if (d->slab) {
+            hipStreamSynchronize(g_stream);
+            hipFree(d->slab);
+            d->slab = nullptr;
+        }
+        if (!hip_ok(hipMalloc(&d->slab, std::max<uint64_t>(need * grid_max, 256)))) {
+            d->slab_total = d->slab_bytes = 0;
+            return PVVC_ERR_HIP;
+        }
+        d->slab_total = need * grid_max;
+        d->slab_bytes = need;
+    }
+    for (auto& sg : b->segs) {
+        if (!sg.count) continue;
+        const uint32_t grid = std::min(sg.count, cap_grid);
+        const uint32_t id = ++d->launch_id == 0 ? ++d->launch_id : d->launch_id;
+        auto ev = g_timing.next();
+        hipEventRecord(ev.first, g_stream);
+        if (W == 2) launch_persistent<2>(b, sg, sink, lds, grid, id, g_stream);
+        else if (W == 4) launch_persistent<4>(b, sg, sink, lds, grid, id, g_stream);
+        else launch_persistent<1>(b, sg, sink, lds, grid, id, g_stream);
+        hipEventRecord(ev.second, g_stream);
+    }
+    return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
+}
+
+}  // extern "C"

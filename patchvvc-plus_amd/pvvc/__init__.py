@@ -1,0 +1,420 @@
+"""pvvc -- MI355X-native patchVVC per-patch GoP decoder (Python host layer).
+
+The decoder itself is HIP (libpvvc_hip.so) behind the C ABI of
+include/pvvc_hip.h; this module wraps that ABI with ctypes and numpy:
+
+* Frame             -- a loaded .frame (io::LoadFrame, src/cuda/common/slice_io.cu:143-275)
+* State, decode_dropin -- the reference's launch_cudaProcess call surface
+                        (include/cuda/render.cuh:117-130) with its device arrays
+* Decoder, Batch    -- native batched decode (per-frame, persistent, graph)
+* synth_stream      -- seeded synthetic humanoid streams (stream writer)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _native
+from ._native import DropinFrame, Points, SynthConfig, Tuning, enc_lib, hip_lib
+
+__all__ = [
+    "STATUS", "Frame", "DeviceBuffer", "State", "Decoder", "Batch", "decode_dropin", "synth_stream",
+    "SYNTH_PRESETS", "Points", "hip_lib", "enc_lib", "MODE_PER_FRAME", "MODE_PERSISTENT", "MODE_GRAPH",
+    "OUT_NATIVE", "OUT_POINTS", "device_info",
+]
+
+STATUS = {
+    0: "OK", 1: "BAD_SLICE", 2: "EMPTY_REFERENCE", 3: "UNMATCHED_SIZE", 4: "ZSTD", 5: "BITSTREAM",
+    6: "SYMBOL_RANGE", 7: "CAPACITY", 8: "INVALID_ARG", 9: "HIP", 10: "TIMEOUT", 11: "IO", 12: "BUSY",
+}
+MODE_PER_FRAME, MODE_PERSISTENT, MODE_GRAPH = 0, 1, 2
+OUT_NATIVE, OUT_POINTS = 0, 1
+POINTS_DTYPE = np.dtype([(n, "<f4") for n in ("x", "y", "z", "r", "g", "b")])
+
+
+class PvvcError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        super().__init__(f"{what}: {STATUS.get(code, code)} ({code})")
+        self.code = code
+
+
+def _check(code: int, what: str) -> None:
+    if code != 0:
+        raise PvvcError(code, what)
+
+
+def device_info() -> dict:
+    L = hip_lib()
+    cu, lds, warp = C.c_int(), C.c_int(), C.c_int()
+    name = C.create_string_buffer(256)
+    _check(L.pvvc_device_info(C.byref(cu), C.byref(lds), C.byref(warp), name, 256), "device_info")
+    return {"cu": cu.value, "lds_per_block": lds.value, "warp": warp.value, "arch": name.value.decode()}
+
+
+# ---------------------------------------------------------------------------
+# Frames
+# ---------------------------------------------------------------------------
+class Frame:
+    """A .frame loaded and zstd-unwrapped on the host."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def load(cls, path: str) -> "Frame":
+        h = C.c_void_p()
+        rc = hip_lib().pvvc_load_frame(str(path).encode(), C.byref(h))
+        if rc != 0:
+            raise PvvcError(rc, f"pvvc_load_frame({path})")
+        return cls(h.value)
+
+    @classmethod
+    def parse(cls, data: bytes) -> "Frame":
+        h = C.c_void_p()
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data) if data else None
+        rc = hip_lib().pvvc_parse_frame(buf, len(data), C.byref(h))
+        if rc != 0:
+            raise PvvcError(rc, "pvvc_parse_frame")
+        return cls(h.value)
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            hip_lib().pvvc_frame_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @property
+    def slice_cnt(self) -> int:
+        return int(hip_lib().pvvc_frame_slice_cnt(self._h))
+
+    @property
+    def points(self) -> int:
+        return int(hip_lib().pvvc_frame_points(self._h))
+
+    @property
+    def timestamp(self) -> int:
+        return int(hip_lib().pvvc_frame_timestamp(self._h))
+
+    def meta(self) -> dict:
+        S = self.slice_cnt
+        out = {
+            "index": np.zeros(S, np.int32), "type": np.zeros(S, np.uint8), "size": np.zeros(S, np.uint32),
+            "qp": np.zeros(S, np.uint8), "mv": np.zeros((S, 16), np.float32),
+            "geometry_size": np.zeros(S, np.uint32), "color_size": np.zeros(S, np.uint32),
+        }
+        hip_lib().pvvc_frame_meta(self._h, *(out[k].ctypes.data for k in
+                                             ("index", "type", "size", "qp", "mv", "geometry_size", "color_size")))
+        return out
+
+    def geometry(self, s: int) -> bytes:
+        m = self.meta()
+        p = hip_lib().pvvc_frame_geometry(self._h, s)
+        return C.string_at(p, int(m["geometry_size"][s])) if p else b""
+
+    def color(self, s: int) -> bytes:
+        m = self.meta()
+        p = hip_lib().pvvc_frame_color(self._h, s)
+        return C.string_at(p, int(m["color_size"][s])) if p else b""
+
+
+# ---------------------------------------------------------------------------
+# Device memory
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """A plain hipMalloc allocation."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = hip_lib().pvvc_dev_alloc(max(1, self.nbytes))
+        if not self.ptr:
+            raise PvvcError(9, f"pvvc_dev_alloc({nbytes})")
+
+    def free(self) -> None:
+        if self.ptr:
+            hip_lib().pvvc_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def zero(self) -> None:
+        _check(hip_lib().pvvc_memset(C.c_void_p(self.ptr), 0, self.nbytes), "memset")
+
+    def read(self, dtype, count: int, offset_bytes: int = 0) -> np.ndarray:
+        a = np.empty(count, dtype=dtype)
+        if a.nbytes:
+            _check(hip_lib().pvvc_memcpy_d2h(a.ctypes.data, C.c_void_p(self.ptr + offset_bytes), a.nbytes), "d2h")
+        return a
+
+    def write(self, arr: np.ndarray, offset_bytes: int = 0) -> None:
+        a = np.ascontiguousarray(arr)
+        _check(hip_lib().pvvc_memcpy_h2d(C.c_void_p(self.ptr + offset_bytes), a.ctypes.data, a.nbytes), "h2d")
+
+
+def synchronize() -> None:
+    _check(hip_lib().pvvc_synchronize(), "synchronize")
+
+
+def last_kernel_ms() -> float:
+    return float(hip_lib().pvvc_last_kernel_ms())
+
+
+def last_launch_times() -> list[float]:
+    L = hip_lib()
+    n = L.pvvc_last_launch_times(None, 0)
+    a = np.zeros(max(n, 1), np.float64)
+    L.pvvc_last_launch_times(a.ctypes.data, n)
+    return a[:n].tolist()
+
+
+# ---------------------------------------------------------------------------
+# Drop-in path
+# ---------------------------------------------------------------------------
+class State:
+    """Device state slots replacing the reference's InvertRAHTOctree array."""
+
+    def __init__(self, n_patches: int, max_points: int):
+        self.n_patches = n_patches
+        self.max_points = max_points
+        self.ptr = hip_lib().pvvc_state_alloc(n_patches, max_points)
+        if not self.ptr:
+            raise PvvcError(hip_lib().pvvc_last_error() or 9, "pvvc_state_alloc")
+
+    def free(self) -> None:
+        if self.ptr:
+            hip_lib().pvvc_state_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def status(self, patch: int) -> int:
+        return int(hip_lib().pvvc_state_patch_status(C.c_void_p(self.ptr), patch))
+
+
+class StagedFrame:
+    """Device arrays of one drop-in call (pvvc_dropin_stage)."""
+
+    def __init__(self, frame: Frame):
+        self.d = DropinFrame()
+        _check(hip_lib().pvvc_dropin_stage(frame.handle, C.byref(self.d)), "pvvc_dropin_stage")
+
+    def release(self) -> None:
+        if self.d.blob:
+            hip_lib().pvvc_dropin_release(C.byref(self.d))
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def launch_dropin(staged: StagedFrame, state: State, out: DeviceBuffer, timestamp: int = 0) -> None:
+    """launch_cudaProcess with the staged arrays (asynchronous)."""
+    d = staged.d
+    hip_lib().launch_cudaProcess(1000, 1, C.c_void_p(out.ptr), timestamp, d.inner_offset, d.index, d.type, d.mv,
+                                 d.size, d.qp, d.geometry, d.geometry_size, d.color, d.color_size,
+                                 C.c_void_p(state.ptr), d.slice_cnt)
+
+
+def decode_dropin(frame: Frame, state: State, out: DeviceBuffer | None = None) -> np.ndarray:
+    """Stages, decodes and reads back one frame as reference Points."""
+    staged = StagedFrame(frame)
+    n = int(staged.d.point_cnt)
+    own = out is None
+    if own:
+        out = DeviceBuffer(max(1, n) * 24)
+    launch_dropin(staged, state, out, frame.timestamp)
+    synchronize()
+    res = out.read(POINTS_DTYPE, n)
+    staged.release()
+    if own:
+        out.free()
+    return res
+
+
+# ---------------------------------------------------------------------------
+# Native decoder
+# ---------------------------------------------------------------------------
+class Batch:
+    def __init__(self, dec: "Decoder", frames: Sequence[Frame]):
+        self.dec = dec
+        arr = (C.c_void_p * len(frames))(*[f.handle.value for f in frames])
+        h = C.c_void_p()
+        _check(hip_lib().pvvc_batch_create(dec.handle, arr, len(frames), C.byref(h)), "pvvc_batch_create")
+        self.handle = h
+        self.n_frames = len(frames)
+        L = hip_lib()
+        self.frame_points = [int(L.pvvc_batch_frame_points(h, f)) for f in range(self.n_frames)]
+        self.frame_offset = [int(L.pvvc_batch_frame_offset(h, f)) for f in range(self.n_frames)]
+        self.total_points = int(sum(self.frame_points))
+        self.input_bytes = int(L.pvvc_batch_input_bytes(h))
+        self.items = int(L.pvvc_batch_items(h))
+
+    def free(self) -> None:
+        if self.handle and self.handle.value:
+            hip_lib().pvvc_batch_free(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def status(self) -> np.ndarray:
+        s = np.zeros(max(1, self.items), np.int32)
+        _check(hip_lib().pvvc_batch_status(self.handle, s.ctypes.data, self.items), "pvvc_batch_status")
+        return s[: self.items]
+
+
+@dataclass
+class Outputs:
+    xyz: DeviceBuffer | None = None
+    rgb: DeviceBuffer | None = None
+    pts: DeviceBuffer | None = None
+    points: int = 0
+
+
+class Decoder:
+    def __init__(self, max_patches: int, max_points: int, tuning: dict | None = None):
+        h = C.c_void_p()
+        _check(hip_lib().pvvc_decoder_create(C.byref(h), max_patches, max_points), "pvvc_decoder_create")
+        self.handle = h
+        if tuning:
+            self.set_tuning(**tuning)
+
+    def close(self) -> None:
+        if self.handle and self.handle.value:
+            hip_lib().pvvc_decoder_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, waves_per_patch=0, lds_bytes=0, wgs_per_cu=0, queue_chunk=0) -> None:
+        t = Tuning(waves_per_patch, lds_bytes, wgs_per_cu, queue_chunk)
+        _check(hip_lib().pvvc_decoder_set_tuning(self.handle, C.byref(t)), "set_tuning")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(hip_lib().pvvc_decoder_get_tuning(self.handle, C.byref(t)), "get_tuning")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def reset(self) -> None:
+        _check(hip_lib().pvvc_decoder_reset(self.handle), "reset")
+
+    def batch(self, frames: Sequence[Frame]) -> Batch:
+        return Batch(self, frames)
+
+    def outputs(self, batch: Batch, output: int = OUT_NATIVE) -> Outputs:
+        n = max(1, batch.total_points)
+        if output == OUT_NATIVE:
+            return Outputs(xyz=DeviceBuffer(12 * n), rgb=DeviceBuffer(3 * n), points=batch.total_points)
+        return Outputs(pts=DeviceBuffer(24 * n), points=batch.total_points)
+
+    def decode(self, batch: Batch, mode: int, out: Outputs) -> None:
+        """Asynchronous decode of the batch into `out`."""
+        out_mode = OUT_NATIVE if out.xyz is not None else OUT_POINTS
+        p = lambda b: C.c_void_p(b.ptr) if b is not None else None  # noqa: E731
+        _check(hip_lib().pvvc_decode_batch(self.handle, batch.handle, mode, out_mode, p(out.xyz), p(out.rgb),
+                                           p(out.pts)), "pvvc_decode_batch")
+
+    def decode_host(self, batch: Batch, mode: int = MODE_PERSISTENT, output: int = OUT_NATIVE):
+        """Decodes and reads back: (xyz f32[N,3], rgb u8[N,3]) or Points[N]."""
+        out = self.outputs(batch, output)
+        self.decode(batch, mode, out)
+        synchronize()
+        n = batch.total_points
+        if output == OUT_NATIVE:
+            return out.xyz.read(np.float32, 3 * n).reshape(n, 3), out.rgb.read(np.uint8, 3 * n).reshape(n, 3)
+        return out.pts.read(POINTS_DTYPE, n)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic streams (stream writer)
+# ---------------------------------------------------------------------------
+SYNTH_PRESETS = {
+    # ~120k voxels/frame, 9-bit, ~1000-point patches, GoP 8, qp 8
+    "c0": dict(grid_bits=9, gop=8, qp_i=8, qp_p=8, n_humanoids=1, jitter=0, patch_min=1100, patch_max=2000,
+               scale=0.85, shell=0.9, patch_mu=float(np.log(1400)), patch_sigma=0.2),
+    # ~800k points/frame, 10-bit, ~800 patches of lognormal 200-3000 points, GoP 8, qp 4, +-1 voxel jitter
+    "c1": dict(grid_bits=10, gop=8, qp_i=4, qp_p=4, n_humanoids=1, jitter=1, patch_min=400, patch_max=3000,
+               scale=0.85, shell=1.5, patch_mu=float(np.log(1500)), patch_sigma=0.7),
+    # ~3M points/frame, 11-bit, heavy-tailed lognormal 50-20000, GoP 16
+    "c2": dict(grid_bits=11, gop=16, qp_i=4, qp_p=4, n_humanoids=1, jitter=1, patch_min=100, patch_max=20000,
+               scale=0.85, shell=1.5, patch_mu=float(np.log(3000)), patch_sigma=1.2),
+    # serving densities: 100k / 9-bit, 400k / 10-bit, 1M / 10-bit
+    "c3_100k": dict(grid_bits=9, gop=8, qp_i=4, qp_p=4, n_humanoids=1, jitter=0, patch_min=400, patch_max=3000,
+                    scale=0.85, shell=0.75, patch_mu=float(np.log(1500)), patch_sigma=0.7),
+    "c3_400k": dict(grid_bits=10, gop=8, qp_i=4, qp_p=4, n_humanoids=1, jitter=0, patch_min=400, patch_max=3000,
+                    scale=0.85, shell=0.75, patch_mu=float(np.log(1500)), patch_sigma=0.7),
+    "c3_1m": dict(grid_bits=10, gop=8, qp_i=4, qp_p=4, n_humanoids=1, jitter=1, patch_min=400, patch_max=3000,
+                  scale=0.85, shell=1.85, patch_mu=float(np.log(1500)), patch_sigma=0.7),
+    # 8 humanoids, 12-bit, ~5M points/frame, GoP 32, qp 1, patches 20-50000
+    "c4": dict(grid_bits=12, gop=32, qp_i=1, qp_p=1, n_humanoids=8, jitter=1, patch_min=40, patch_max=50000,
+               scale=0.85, shell=0.7, patch_mu=float(np.log(4000)), patch_sigma=1.4),
+}
+
+
+def synth_config(preset: str = "c0", n_frames: int | None = None, seed: int = 1, threads: int = 8,
+                 zstd_level: int = 3, **over) -> SynthConfig:
+    p = dict(SYNTH_PRESETS[preset])
+    p.update(over)
+    cfg = SynthConfig()
+    cfg.grid_bits = p["grid_bits"]
+    cfg.n_frames = n_frames if n_frames is not None else p["gop"]
+    cfg.gop = p["gop"]
+    cfg.qp_i, cfg.qp_p = p["qp_i"], p["qp_p"]
+    cfg.n_humanoids = p["n_humanoids"]
+    cfg.jitter = p["jitter"]
+    cfg.patch_min, cfg.patch_max = p["patch_min"], p["patch_max"]
+    cfg.zstd_level = zstd_level
+    cfg.threads = threads
+    cfg.scale, cfg.shell = p["scale"], p["shell"]
+    cfg.patch_mu, cfg.patch_sigma = p["patch_mu"], p["patch_sigma"]
+    cfg.noise_sigma = p.get("noise_sigma", 4.0)
+    cfg.max_rot_deg = p.get("max_rot_deg", 2.0)
+    cfg.max_trans = p.get("max_trans", 2.0)
+    cfg.seed = seed
+    return cfg
+
+
+def synth_stream(preset: str = "c0", n_frames: int | None = None, seed: int = 1, threads: int = 8,
+                 zstd_level: int = 3, **over) -> list[bytes]:
+    """Generates a seeded synthetic stream; returns the .frame images."""
+    L = enc_lib()
+    cfg = synth_config(preset, n_frames, seed, threads, zstd_level, **over)
+    h = C.c_void_p()
+    _check(L.pvvc_synth_generate(C.byref(cfg), C.byref(h)), "pvvc_synth_generate")
+    try:
+        out = []
+        for i in range(L.pvvc_stream_frames(h)):
+            n = C.c_size_t()
+            p = L.pvvc_stream_frame_bytes(h, i, C.byref(n))
+            out.append(C.string_at(p, n.value))
+        return out
+    finally:
+        L.pvvc_stream_free(h)

@@ -1,0 +1,133 @@
+"""ctypes bindings of the package's native libraries (include/pvvc_hip.h).
+
+libpvvc_hip.so holds the HIP decoder; libpvvc_enc.so the stream writer.
+Loading fails loudly when a library is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+_LIB_DIR = Path(__file__).resolve().parent / "_lib"
+
+c_u8p = C.POINTER(C.c_uint8)
+c_i32p = C.POINTER(C.c_int)
+c_u32p = C.POINTER(C.c_uint32)
+c_f32p = C.POINTER(C.c_float)
+c_i64p = C.POINTER(C.c_int64)
+c_vp = C.c_void_p
+
+
+class Points(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("x", "y", "z", "r", "g", "b")]
+
+
+class DropinFrame(C.Structure):
+    """pvvc_dropin_frame: device arrays of one launch_cudaProcess call."""
+    _fields_ = [
+        ("inner_offset", c_vp), ("index", c_vp), ("type", c_vp), ("mv", c_vp), ("size", c_vp),
+        ("qp", c_vp), ("geometry", c_vp), ("geometry_size", c_vp), ("color", c_vp), ("color_size", c_vp),
+        ("slice_cnt", C.c_int), ("point_cnt", C.c_int64), ("blob", c_vp), ("blob_bytes", C.c_size_t),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("waves_per_patch", C.c_int), ("lds_bytes", C.c_int), ("wgs_per_cu", C.c_int),
+                ("queue_chunk", C.c_int)]
+
+
+class SynthConfig(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("grid_bits", "n_frames", "gop", "qp_i", "qp_p", "n_humanoids", "jitter",
+                                       "patch_min", "patch_max", "zstd_level", "threads")] + \
+               [(n, C.c_double) for n in ("scale", "shell", "patch_mu", "patch_sigma", "noise_sigma", "max_rot_deg",
+                                          "max_trans")] + [("seed", C.c_uint64)]
+
+
+def _sig(lib, name, restype, *argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+
+
+_hip = None
+_enc = None
+
+
+def hip_lib() -> C.CDLL:
+    """The HIP decoder library; raises if it has not been built."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    path = _LIB_DIR / "libpvvc_hip.so"
+    if not path.exists():
+        raise RuntimeError(f"HIP decoder library missing: {path} (run patchvvc-plus_amd/build.py)")
+    L = C.CDLL(str(path))
+    _sig(L, "pvvc_load_frame", C.c_int, C.c_char_p, C.POINTER(c_vp))
+    _sig(L, "pvvc_parse_frame", C.c_int, c_vp, C.c_size_t, C.POINTER(c_vp))
+    _sig(L, "pvvc_frame_free", None, c_vp)
+    _sig(L, "pvvc_frame_timestamp", C.c_int, c_vp)
+    _sig(L, "pvvc_frame_slice_cnt", C.c_uint32, c_vp)
+    _sig(L, "pvvc_frame_points", C.c_uint64, c_vp)
+    _sig(L, "pvvc_frame_meta", None, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp)
+    _sig(L, "pvvc_frame_geometry", c_vp, c_vp, C.c_uint32)
+    _sig(L, "pvvc_frame_color", c_vp, c_vp, C.c_uint32)
+    _sig(L, "pvvc_state_alloc", c_vp, C.c_int, C.c_int)
+    _sig(L, "pvvc_state_free", None, c_vp)
+    _sig(L, "pvvc_state_patch_status", C.c_int, c_vp, C.c_int)
+    _sig(L, "launch_cudaProcess", None, C.c_int, C.c_int, c_vp, C.c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+         c_vp, c_vp, c_vp, c_vp, C.c_int)
+    _sig(L, "pvvc_set_stream", C.c_int, c_vp)
+    _sig(L, "pvvc_last_error", C.c_int)
+    _sig(L, "pvvc_set_white_small_patches", C.c_int, C.c_int)
+    _sig(L, "pvvc_set_dropin_tuning", C.c_int, C.c_int, C.c_int)
+    _sig(L, "pvvc_dropin_stage", C.c_int, c_vp, C.POINTER(DropinFrame))
+    _sig(L, "pvvc_dropin_release", None, C.POINTER(DropinFrame))
+    _sig(L, "pvvc_decoder_create", C.c_int, C.POINTER(c_vp), C.c_int, C.c_int)
+    _sig(L, "pvvc_decoder_destroy", C.c_int, c_vp)
+    _sig(L, "pvvc_decoder_set_tuning", C.c_int, c_vp, C.POINTER(Tuning))
+    _sig(L, "pvvc_decoder_get_tuning", C.c_int, c_vp, C.POINTER(Tuning))
+    _sig(L, "pvvc_decoder_reset", C.c_int, c_vp)
+    _sig(L, "pvvc_batch_create", C.c_int, c_vp, C.POINTER(c_vp), C.c_int, C.POINTER(c_vp))
+    _sig(L, "pvvc_batch_free", None, c_vp)
+    _sig(L, "pvvc_batch_frames", C.c_int, c_vp)
+    _sig(L, "pvvc_batch_items", C.c_int, c_vp)
+    _sig(L, "pvvc_batch_frame_points", C.c_int64, c_vp, C.c_int)
+    _sig(L, "pvvc_batch_frame_offset", C.c_int64, c_vp, C.c_int)
+    _sig(L, "pvvc_batch_input_bytes", C.c_int64, c_vp)
+    _sig(L, "pvvc_batch_status", C.c_int, c_vp, c_vp, C.c_int)
+    _sig(L, "pvvc_decode_batch", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp)
+    _sig(L, "pvvc_dev_alloc", c_vp, C.c_size_t)
+    _sig(L, "pvvc_dev_free", None, c_vp)
+    _sig(L, "pvvc_memcpy_h2d", C.c_int, c_vp, c_vp, C.c_size_t)
+    _sig(L, "pvvc_memcpy_d2h", C.c_int, c_vp, c_vp, C.c_size_t)
+    _sig(L, "pvvc_memset", C.c_int, c_vp, C.c_int, C.c_size_t)
+    _sig(L, "pvvc_synchronize", C.c_int)
+    _sig(L, "pvvc_last_kernel_ms", C.c_double)
+    _sig(L, "pvvc_last_launch_times", C.c_int, c_vp, C.c_int)
+    _sig(L, "pvvc_device_info", C.c_int, c_i32p, c_i32p, c_i32p, C.c_char_p, C.c_int)
+    _hip = L
+    return L
+
+
+def enc_lib() -> C.CDLL:
+    """The stream-writer library (host only)."""
+    global _enc
+    if _enc is not None:
+        return _enc
+    path = _LIB_DIR / "libpvvc_enc.so"
+    if not path.exists():
+        raise RuntimeError(f"stream writer library missing: {path} (run patchvvc-plus_amd/build.py)")
+    L = C.CDLL(str(path))
+    _sig(L, "pvvc_synth_generate", C.c_int, C.POINTER(SynthConfig), C.POINTER(c_vp))
+    _sig(L, "pvvc_stream_frames", C.c_int, c_vp)
+    _sig(L, "pvvc_stream_frame_bytes", c_vp, c_vp, C.c_int, C.POINTER(C.c_size_t))
+    _sig(L, "pvvc_stream_frame_points", C.c_uint64, c_vp, C.c_int)
+    _sig(L, "pvvc_stream_write_frame", C.c_int, c_vp, C.c_int, C.c_char_p)
+    _sig(L, "pvvc_stream_free", None, c_vp)
+    _sig(L, "pvvc_enc_octree", C.c_int64, c_vp, C.c_int, c_vp, C.c_int64, c_i32p)
+    _sig(L, "pvvc_enc_raht", C.c_int64, c_vp, C.c_int, c_vp, c_vp)
+    _sig(L, "pvvc_enc_rlgr", C.c_int64, c_vp, C.c_int64, c_vp, C.c_int64)
+    _sig(L, "pvvc_enc_gop_patch", C.c_int64, c_vp, C.c_int, c_vp, C.c_int, C.c_int, C.c_int, C.c_int, c_vp,
+         C.c_int64)
+    _enc = L
+    return L

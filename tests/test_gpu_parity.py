@@ -1,0 +1,86 @@
+"""GPU parity: the HIP decoder against the CPU oracle on the same streams.
+
+Bar: geometry (xyz after motion) bit-exact as float values; colours of the
+native path bit-exact RGB8 (the stated bar is |diff| <= 1, asserted too);
+drop-in Points colours equal to the oracle's unrounded FP32 RGB / 255.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_decode(oracle, paths, threads=8):
+    dec = oracle.StreamDecoder(fp64=True)
+    outs = []
+    for p in paths:
+        f = oracle.OracleFrame(str(p))
+        assert f.rc == 0
+        outs.append(dec.decode(f, threads))
+    return outs
+
+
+@pytest.fixture(scope="module")
+def c0_oracle(oracle, c0_files):
+    return _oracle_decode(oracle, c0_files)
+
+
+def test_dropin_c0_matches_oracle(pvvc, c0_files, c0_oracle):
+    _say("dropin c0: frames", len(c0_files))
+    frames = [pvvc.Frame.load(str(p)) for p in c0_files]
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames)
+    st = pvvc.State(n_patches, max_pts)
+    for i, (f, ref) in enumerate(zip(frames, c0_oracle)):
+        _say("  frame", i, "points", f.points)
+        pts = pvvc.decode_dropin(f, st)
+        assert (ref["status"] == 0).all()
+        for s in range(f.slice_cnt):
+            assert st.status(int(f.meta()["index"][s])) == 0
+        xyz = np.stack([pts["x"], pts["y"], pts["z"]], 1)
+        np.testing.assert_array_equal(xyz, ref["xyz"])
+        rgbf = np.stack([pts["r"], pts["g"], pts["b"]], 1)
+        np.testing.assert_array_equal(rgbf, ref["rgbf"] / np.float32(255))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["per_frame", "persistent", "graph"])
+def test_native_c0_matches_oracle(pvvc, c0_files, c0_oracle, mode):
+    _say("native c0 mode", mode)
+    frames = [pvvc.Frame.load(str(p)) for p in c0_files]
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames)
+    dec = pvvc.Decoder(n_patches, max_pts)
+    b = dec.batch(frames)
+    xyz, rgb = dec.decode_host(b, mode, pvvc.OUT_NATIVE)
+    assert (b.status() == 0).all(), np.unique(b.status())
+    for i, ref in enumerate(c0_oracle):
+        o, n = b.frame_offset[i], b.frame_points[i]
+        np.testing.assert_array_equal(xyz[o:o + n], ref["xyz"])
+        d = np.abs(rgb[o:o + n].astype(int) - ref["rgb"].astype(int))
+        assert d.max() <= 1
+        np.testing.assert_array_equal(rgb[o:o + n], ref["rgb"])
+
+
+@pytest.mark.parametrize("waves,lds", [(1, 4096), (2, 48 * 1024), (4, 16384)])
+def test_tuning_variants_identical(pvvc, c0_files, c0_oracle, waves, lds):
+    """waves-per-patch and LDS budgets (small budgets spill to the global
+    workspace) give identical output."""
+    _say("tuning", waves, lds)
+    frames = [pvvc.Frame.load(str(p)) for p in c0_files]
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames)
+    dec = pvvc.Decoder(n_patches, max_pts, tuning=dict(waves_per_patch=waves, lds_bytes=lds))
+    b = dec.batch(frames)
+    xyz, rgb = dec.decode_host(b, pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+    assert (b.status() == 0).all()
+    for i, ref in enumerate(c0_oracle):
+        o, n = b.frame_offset[i], b.frame_points[i]
+        np.testing.assert_array_equal(xyz[o:o + n], ref["xyz"])
+        np.testing.assert_array_equal(rgb[o:o + n], ref["rgb"])
