@@ -86,6 +86,10 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
                         uint32_t* geometry_size, uint8_t** color, uint32_t* color_size, pvvc_state* invertRAHTOctree_gpu,
                         int patch_size);
 
+/* Launch shape of launch_cudaProcess: wavefronts per slice (1, 2, 4) and the
+ * LDS workspace per slice in bytes (arrays that do not fit use the state
+ * slot's global workspace). */
+int pvvc_set_dropin_tuning(int waves, int lds_bytes);
 /* Stream used by launch_cudaProcess and the native calls (a hipStream_t). */
 int pvvc_set_stream(void* stream);
 /* Last error recorded by any call in this process (sticky until read). */
@@ -168,6 +172,7 @@ int pvvc_batch_frames(const pvvc_batch* batch);
 int64_t pvvc_batch_frame_points(const pvvc_batch* batch, int frame);
 int64_t pvvc_batch_frame_offset(const pvvc_batch* batch, int frame);
 int64_t pvvc_batch_input_bytes(const pvvc_batch* batch);
+int pvvc_batch_items(const pvvc_batch* batch);
 
 /* Output modes. */
 enum pvvc_output { PVVC_OUT_NATIVE = 0, PVVC_OUT_POINTS = 1 };
@@ -201,6 +206,9 @@ int pvvc_synchronize(void);
 /* Kernel time of the last decode call, from HIP events recorded on the
  * library stream around its launches (milliseconds). */
 double pvvc_last_kernel_ms(void);
+/* Per-launch kernel times (ms) of the last decode call; returns the launch
+ * count (fills at most `max`). */
+int pvvc_last_launch_times(double* ms, int max);
 /* Device properties used for launch shapes: CU count, LDS per workgroup. */
 int pvvc_device_info(int* cu_count, int* lds_per_block, int* warp_size, char* name, int name_len);
 

@@ -128,16 +128,147 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
     return static_cast<int32_t>(h ^ (0u - static_cast<uint32_t>(u & 1u)));
 }
 
+/* Common-case RLGR symbols as a tight scalar loop (entropy_codec.cc:207-283):
+ *  - window refill from the VGPR chunk `cur` (v_readlane), 32 bits at a time;
+ *  - k = 0: Golomb-Rice code (unary prefix by one leading-ones count,
+ *    remainder by shifts), k_P / k_RP adaptation (:255-276);
+ *  - k > 0 with no 1-bits in the run prefix: the run length in k bits, the
+ *    zero run (only advances the symbol index), then the Golomb-Rice coded
+ *    symbol + 1 and the adaptation of :240-251;
+ *  - each decoded symbol is written into lane `nrel` of `acc`.
+ * Every case is checked before any bit is consumed; the loop returns to the
+ * caller, with the state exactly at a symbol boundary, on: chunk end
+ * (nrel == lim), a chunk rotation of `cur`, an escape or a unary prefix of 32+,
+ * k_R > 24, a code not entirely inside the 64-bit window, 1-bits in a run
+ * prefix, or a zero run reaching the chunk end.  s80-s95 are scratch; no
+ * memory operations. */
+__device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nb, uint32_t& wpos, uint32_t& kRP, uint32_t& kP,
+                                              uint32_t& nrel, uint32_t lim, uint32_t cur, int32_t& acc, uint32_t lane) {
+    int32_t vt;
+    asm volatile(
+        "1:\n\t"
+        "s_cmp_ge_u32 %[nrel], %[lim]\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        /* refill: 32 more bits when at most 32 remain */
+        "s_cmp_gt_i32 %[nb], 32\n\t"
+        "s_cbranch_scc1 2f\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 9f\n\t"                   /* next chunk needed */
+        "s_nop 3\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, 32, %[nb]\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nb], %[nb], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "2:\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc1 3f\n\t"
+        /* ---- k = 0 ---- */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"      /* ones; -1 when the window is all ones */
+        "s_cmp_gt_u32 s84, 31\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshr_b32 s85, %[kRP], 2\n\t"           /* k_R */
+        "s_cmp_gt_u32 s85, 24\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
+        "s_add_u32 s87, s86, s85\n\t"             /* code length */
+        "s_cmp_gt_i32 s87, %[nb]\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s86\n\t"
+        "s_lshl_b64 %[win], s[80:81], s85\n\t"
+        "s_sub_i32 %[nb], %[nb], s87\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], 1\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[82:83], s88\n\t" /* remainder */
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_or_b32 s89, s89, s82\n\t"              /* u */
+        "s_cmp_eq_u32 s89, 0\n\t"
+        "s_cselect_b32 s88, 3, -1\n\t"
+        "s_add_i32 %[kP], %[kP], s88\n\t"
+        "s_max_i32 %[kP], %[kP], 0\n\t"           /* k_P: u ? max(k_P-1,0) : k_P+3 */
+        "s_branch 5f\n\t"
+        /* ---- k > 0: run, then symbol + 1 ---- */
+        "3:\n\t"
+        "s_lshr_b32 s85, %[kP], 2\n\t"            /* k */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_cbranch_scc1 9f\n\t"                   /* 1-bits in the run prefix */
+        "s_lshl_b64 s[80:81], %[win], 1\n\t"
+        "s_sub_u32 s88, 64, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t" /* m = next k bits */
+        "s_lshl_b64 s[92:93], s[80:81], s85\n\t" /* window after the run code */
+        "s_add_u32 s87, s85, 1\n\t"               /* run code length */
+        "s_add_u32 s86, %[nrel], s82\n\t"         /* index after the zeros */
+        "s_cmp_ge_u32 s86, %[lim]\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_not_b64 s[80:81], s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_cmp_gt_u32 s84, 31\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshr_b32 s85, %[kRP], 2\n\t"           /* k_R */
+        "s_cmp_gt_u32 s85, 24\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_add_u32 s94, s84, 1\n\t"
+        "s_add_u32 s95, s94, s85\n\t"
+        "s_add_u32 s95, s95, s87\n\t"             /* run code + symbol code */
+        "s_cmp_gt_i32 s95, %[nb]\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s94\n\t"
+        "s_lshl_b64 %[win], s[80:81], s85\n\t"
+        "s_sub_i32 %[nb], %[nb], s95\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], 1\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[82:83], s88\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
+        "s_add_u32 s89, s89, 1\n\t"               /* u + 1 */
+        "s_sub_i32 %[kP], %[kP], 1\n\t"
+        "s_max_i32 %[kP], %[kP], 0\n\t"
+        "s_mov_b32 %[nrel], s86\n\t"
+        /* ---- common tail: k_RP, symbol, lane write ---- */
+        "5:\n\t"
+        "s_add_i32 s88, %[kRP], s84\n\t"
+        "s_add_i32 s88, s88, -1\n\t"
+        "s_min_u32 s88, s88, 128\n\t"
+        "s_sub_i32 %[kRP], %[kRP], 2\n\t"
+        "s_max_i32 %[kRP], %[kRP], 0\n\t"
+        "s_cmp_eq_u32 s84, 0\n\t"
+        "s_cselect_b32 %[kRP], %[kRP], s88\n\t"  /* k_RP: p ? min(k_RP+p-1,128) : max(k_RP-2,0) */
+        "s_lshr_b32 s88, s89, 1\n\t"
+        "s_bfe_i32 s89, s89, 0x10000\n\t"
+        "s_xor_b32 s88, s88, s89\n\t"             /* Unsign2Sign */
+        "v_mov_b32 %[vt], s88\n\t"
+        "v_cmp_eq_u32 vcc, %[nrel], %[lane]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_add_u32 %[nrel], %[nrel], 1\n\t"
+        "s_branch 1b\n\t"
+        "9:\n\t"
+        : [win] "+s"(win), [nb] "+s"(nb), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [nrel] "+s"(nrel),
+          [acc] "+v"(acc), [vt] "=&v"(vt)
+        : [lim] "s"(lim), [cur] "v"(cur), [lane] "v"(lane)
+        : "vcc", "scc", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92",
+          "s93", "s94", "s95");
+}
+
 /* Decodes 3n symbols into coef[0, 3n) (planar Y, U, V) dequantised with qp.
- * Called by all 64 lanes of one wavefront; control flow is wave-uniform. */
-__device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src, uint32_t nbytes, uint32_t total, uint32_t qp,
-                                float* coef) {
+ * Called by all 64 lanes of one wavefront; control flow is wave-uniform.
+ * Symbols are gathered 64 at a time in `acc` (lane j = symbol base + j);
+ * zero runs only advance the symbol index. */
+__device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src, uint32_t nbytes, uint32_t total,
+                                                uint32_t qp, float* coef) {
     BitReader br;
     br_init(br, src, nbytes);
+    br_refill(br);
+    br_refill(br);
     const uint32_t lane = lane_id();
     int32_t acc = 0;
-    uint32_t acc_base = 0;
-    uint32_t n = 0, kP = 0, kRP = 8;
+    uint32_t acc_base = 0, nrel = 0;
+    uint32_t kP = 0, kRP = 8;
     int st = PVVC_OK;
 
     auto store = [&](uint32_t base, int32_t v) {
@@ -145,22 +276,31 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
         if (i < total) coef[i] = static_cast<float>(static_cast<long long>(v) * static_cast<long long>(qp));
     };
     auto emit = [&](int32_t s) {
-        if (n - acc_base >= 64u) {
-            store(acc_base, acc);
-            acc = 0;
-            const uint32_t nbase = n & ~63u;
-            for (uint32_t b = acc_base + 64u; b < nbase; b += 64u) store(b, 0);
-            acc_base = nbase;
-        }
-        acc = (lane == n - acc_base) ? s : acc;
-        ++n;
+        acc = (lane == nrel) ? s : acc;
+        ++nrel;
     };
 
-    while (n < total) {
-        uint32_t k = kP >> 2;
+    while (acc_base < total) {
+        const uint32_t rest = total - acc_base;
+        const uint32_t lim = rest < 64u ? rest : 64u;
+        /* the adaptation state is wave-uniform; say so for the SGPR operands */
+        kRP = __builtin_amdgcn_readfirstlane(kRP);
+        kP = __builtin_amdgcn_readfirstlane(kP);
+        rlgr_fast_run(br.win, br.nb, br.wpos, kRP, kP, nrel, lim, br.cur, acc, lane);
+        if (nrel >= lim) { /* chunk of 64 symbols complete */
+            store(acc_base, acc);
+            acc = 0;
+            acc_base += 64u;
+            nrel = 0;
+            continue;
+        }
+        if (br.nb <= 32) {
+            br_refill(br);
+            continue;
+        }
         const uint32_t kR = kRP >> 2;
-        if (k == 0) {
-            /* no run-length coding (:255-276) */
+        if (kP < 4) {
+            /* k = 0 symbol the fast loop declined: escape, long prefix, large k_R */
             const uint64_t u = br_gr(br, kR);
             if (u >= (1ull << 32)) {
                 st = PVVC_ERR_SYMBOL_RANGE;
@@ -169,50 +309,64 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
             emit(unsign2sign(u));
             kRP = adapt_krp(kRP, u >> kR);
             kP = u ? (kP ? kP - 1 : 0u) : kP + 3;
-        } else {
-            /* run-length coding (:221-252) */
-            uint64_t m = 0;
-            bool bad = false;
-            for (;;) {
-                br_refill(br);
-                const uint32_t t = leading_ones(br.win);
-                for (uint32_t i = 0; i < t; ++i) {
-                    m += 1ull << k;
-                    kP += 2;
-                    k = kP >> 2;
-                    if (k > 30) {
-                        bad = true;
-                        break;
-                    }
-                }
-                if (bad) break;
-                if (static_cast<int32_t>(t) < br.nb) {
-                    br_skip(br, t + 1);
+            continue;
+        }
+        /* run-length coding (entropy_codec.cc:221-252) */
+        uint32_t k = kP >> 2;
+        uint64_t m = 0;
+        bool bad = false;
+        for (;;) {
+            br_refill(br);
+            const uint32_t t = leading_ones(br.win);
+            for (uint32_t i = 0; i < t; ++i) {
+                m += 1ull << k;
+                kP += 2;
+                k = kP >> 2;
+                if (k > 30) {
+                    bad = true;
                     break;
                 }
-                br_skip(br, t);
             }
-            if (bad) {
-                st = PVVC_ERR_BITSTREAM;
+            if (bad) break;
+            if (static_cast<int32_t>(t) < br.nb) {
+                br_skip(br, t + 1);
                 break;
             }
-            m += br_get(br, k);
-            if (m >= static_cast<uint64_t>(total - n)) {
-                n = total;
-                break;
-            }
-            n += static_cast<uint32_t>(m);
-            const uint64_t u = br_gr(br, kR);
-            if (u + 1 >= (1ull << 32)) {
-                st = PVVC_ERR_SYMBOL_RANGE;
-                break;
-            }
-            emit(unsign2sign(u + 1));
-            kRP = adapt_krp(kRP, u >> kR);
-            kP = kP ? kP - 1 : 0u;
+            br_skip(br, t);
         }
+        if (bad) {
+            st = PVVC_ERR_BITSTREAM;
+            break;
+        }
+        m += br_get(br, k);
+        const uint64_t n = static_cast<uint64_t>(acc_base) + nrel;
+        if (m >= total - n) { /* the run reaches the end */
+            store(acc_base, acc);
+            acc = 0;
+            acc_base += 64u;
+            nrel = 0;
+            break;
+        }
+        const uint32_t n2 = static_cast<uint32_t>(n + m);
+        if (n2 - acc_base >= 64u) { /* the run leaves this chunk */
+            store(acc_base, acc);
+            acc = 0;
+            const uint32_t nb2 = n2 & ~63u;
+            for (uint32_t b = acc_base + 64u; b < nb2; b += 64u) store(b, 0);
+            acc_base = nb2;
+        }
+        nrel = n2 - acc_base;
+        const uint64_t u = br_gr(br, kR);
+        if (u + 1 >= (1ull << 32)) {
+            st = PVVC_ERR_SYMBOL_RANGE;
+            break;
+        }
+        emit(unsign2sign(u + 1));
+        kRP = adapt_krp(kRP, u >> kR);
+        kP = kP ? kP - 1 : 0u;
     }
-    store(acc_base, acc);
+    /* the chunk in flight and everything after it */
+    if (acc_base < total) store(acc_base, acc);
     for (uint32_t b = acc_base + 64u; b < total; b += 64u) store(b, 0);
     const uint64_t consumed = static_cast<uint64_t>(br.wpos) * 32u - static_cast<uint64_t>(br.nb);
     if (st == PVVC_OK && consumed > static_cast<uint64_t>(nbytes) * 8u) st = PVVC_ERR_BITSTREAM;
@@ -273,7 +427,7 @@ __device__ __forceinline__ int tree_structure(const uint8_t* __restrict__ geo, u
             meta.cnt[l + 1] = cnt;
             meta.base[l + 1] = base;
         }
-        if (l + 2 < h) wmax = cnt > wmax ? cnt : wmax;
+        if (l + 3 < h) wmax = cnt > wmax ? cnt : wmax; /* levels stored between passes: 1 .. h-3 */
     }
     if (lane == 0) {
         meta.height = h;
@@ -317,10 +471,28 @@ __device__ __forceinline__ uint8_t to_u8(float f) {
     return static_cast<uint8_t>(q);
 }
 
+/* Reference Points colour: unrounded RGB / 255 (CUDA2OpenGL.cu:49-51); the
+ * white-small-patch artefact (:54-58) only on request. */
+__device__ __forceinline__ void write_points_rgb(Points* pt, float r, float g, float b, uint32_t n, int white_small) {
+    if (white_small && n < 10) {
+        pt->r = pt->g = pt->b = 1.0f;
+    } else {
+        pt->r = r / 255.0f;
+        pt->g = g / 255.0f;
+        pt->b = b / 255.0f;
+    }
+}
+
 __device__ __forceinline__ void emit_leaf(uint32_t c, float cx, float cy, float cz, float Y, float U, float V,
                                           const SliceArgs& a, float* ref, uint32_t ref_stride, bool predict,
                                           const Sink& sink) {
-    if (predict) { /* invert_raht_octree.cc:304-308 */
+    const bool deferred = predict && a.stage != nullptr;
+    if (deferred) { /* reference added after the launch (finish_kernel) */
+        float* sp = a.stage + 3 * (a.out_off + c);
+        sp[0] = Y;
+        sp[1] = U;
+        sp[2] = V;
+    } else if (predict) { /* invert_raht_octree.cc:304-308 */
         Y = Y + ref[c];
         U = U + ref[ref_stride + c];
         V = V + ref[2 * ref_stride + c];
@@ -342,23 +514,18 @@ __device__ __forceinline__ void emit_leaf(uint32_t c, float cx, float cy, float 
         px[0] = x;
         px[1] = y;
         px[2] = z;
+        if (deferred) return;
         uint8_t* pc = sink.rgb + 3 * o;
         pc[0] = to_u8(r);
         pc[1] = to_u8(g);
         pc[2] = to_u8(b);
     } else {
-        Points p;
-        p.x = x;
-        p.y = y;
-        p.z = z;
-        if (sink.white_small && a.n < 10) {
-            p.r = p.g = p.b = 1.0f;
-        } else {
-            p.r = r / 255.0f;
-            p.g = g / 255.0f;
-            p.b = b / 255.0f;
-        }
-        sink.pts[o] = p;
+        Points* pt = sink.pts + o;
+        pt->x = x;
+        pt->y = y;
+        pt->z = z;
+        if (deferred) return;
+        write_points_rgb(pt, r, g, b, a.n, sink.white_small);
     }
 }
 
@@ -375,6 +542,77 @@ __device__ __forceinline__ void inv_haar(uint32_t w0, uint32_t w1, float g, floa
 /* ======================================================================== */
 /* Device: one slice, one workgroup                                          */
 /* ======================================================================== */
+
+/* Optional per-phase time stamps (built with -DPVVC_PHASE_PROFILE): for
+ * item i, stamps [8i, 8i+8) = start, after tree, after RLGR, after the
+ * intra wait, after the top-down pass, end, and the XCC id; device-wide
+ * 100 MHz clock (s_memrealtime). */
+#ifdef PVVC_PHASE_PROFILE
+__device__ uint64_t* g_prof = nullptr;
+#define PVVC_STAMP(item, k)                                              \
+    do {                                                                 \
+        if (g_prof && threadIdx.x == 0) g_prof[8ull * (item) + (k)] = wall_clock64(); \
+    } while (0)
+#else
+#define PVVC_STAMP(item, k) \
+    do {                    \
+    } while (0)
+#endif
+
+/* Inverse hierarchical transform of one node (octree_base.cc:84-100,
+ * 186-206): child weights cw[i] (0 = empty child), the node's DC, and its
+ * ACs at reversed stream positions r, r+1, ... (coef is planar Y|U|V of n
+ * coefficients in stream order).  Writes the 8 child DCs (meaningful for
+ * occupied children).  FP32 in the reference's operation order; where one
+ * side of a pair is empty the transform is the identity on the other side. */
+__device__ __forceinline__ void node_inverse(const uint32_t cw[8], float dY, float dU, float dV, const float* coef,
+                                             uint32_t n, uint32_t r, float oY[8], float oU[8], float oV[8]) {
+    uint32_t w[16];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[8 + i] = cw[i];
+    w[4] = w[8] + w[12];
+    w[5] = w[9] + w[13];
+    w[6] = w[10] + w[14];
+    w[7] = w[11] + w[15];
+    w[2] = w[4] + w[6];
+    w[3] = w[5] + w[7];
+    float cY[16], cU[16], cV[16];
+    cY[1] = dY;
+    cU[1] = dU;
+    cV[1] = dV;
+    constexpr int NW[8][2] = {{0, 0}, {2, 3}, {4, 6}, {5, 7}, {8, 12}, {9, 13}, {10, 14}, {11, 15}};
+#pragma unroll
+    for (int idx = 1; idx < 8; ++idx) {
+        const int ia = NW[idx][0], ib = NW[idx][1];
+        const uint32_t w0 = w[ia], w1 = w[ib];
+        if (w0 != 0 && w1 != 0) {
+            const float hY = coef[n - 1 - r], hU = coef[2 * n - 1 - r], hV = coef[3 * n - 1 - r];
+            ++r;
+            const float base = sqrtf(static_cast<float>(w0 + w1));
+            const float ta = sqrtf(static_cast<float>(w0)) / base;
+            const float tc = sqrtf(static_cast<float>(w1)) / base;
+            cY[ia] = cY[idx] * ta + hY * (-tc);
+            cY[ib] = cY[idx] * tc + hY * ta;
+            cU[ia] = cU[idx] * ta + hU * (-tc);
+            cU[ib] = cU[idx] * tc + hU * ta;
+            cV[ia] = cV[idx] * ta + hV * (-tc);
+            cV[ib] = cV[idx] * tc + hV * ta;
+        } else {
+            cY[ia] = cY[idx];
+            cU[ia] = cU[idx];
+            cV[ia] = cV[idx];
+            cY[ib] = cY[idx];
+            cU[ib] = cU[idx];
+            cV[ib] = cV[idx];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        oY[i] = cY[8 + i];
+        oU[i] = cU[8 + i];
+        oV[i] = cV[8 + i];
+    }
+}
 
 struct Arena {
     uint8_t* lds;
@@ -397,16 +635,25 @@ struct Arena {
         }
         return nullptr;
     }
+    __device__ void* take_global(uint64_t bytes) {
+        bytes = (bytes + 255u) & ~uint64_t(255);
+        if (g_off + bytes > gws_bytes) return nullptr;
+        void* p = gws + g_off;
+        g_off += bytes;
+        return p;
+    }
 };
 
 template <int WAVES>
 __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar, const Sink& sink, LevelMeta& meta,
-                            uint32_t launch_id, bool allow_wait) {
+                            uint32_t launch_id, bool allow_wait, uint32_t item) {
+    PVVC_STAMP(item, 0);
     constexpr uint32_t NT = 64u * WAVES;
     const uint32_t tid = threadIdx.x;
-    const uint32_t wave = tid >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6); /* uniform per wavefront */
     const bool predict = (a.type & PVVC_TYPE_PREDICT) != 0;
-    const bool wait = predict && allow_wait && a.wait;
+    const bool wait = predict && allow_wait && a.wait; /* deferred reference add */
+    if (!wait) a.stage = nullptr;
     int st = PVVC_OK;
 
     /* ---- validation (SetSlice, invert_raht_octree.cc:25-40) ---- */
@@ -424,6 +671,8 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
             a.geo_size = slot->geo_size;
         }
     } else if (a.n > slot->n_cap || !a.geo || a.geo_size < 25) {
+        /* deferred: the intra slice decodes in this launch; validity against
+         * it is settled by the finish pass */
         st = PVVC_ERR_CAPACITY;
     }
     if (st == PVVC_OK && a.n == 0) st = PVVC_ERR_UNMATCHED_SIZE;
@@ -433,6 +682,8 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
     uint32_t* wt = nullptr;
     if (st == PVVC_OK) {
         const uint64_t imax = a.geo_size - 25u;
+        /* LDS first; patches too large for the per-item LDS budget stream
+         * their coefficients through the global workspace */
         coef = static_cast<float*>(ar.take(12ull * a.n));
         fc = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
         wt = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
@@ -445,8 +696,10 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
             int ts = tree_structure(a.geo, a.geo_size, a.n, fc, meta);
             __syncthreads();
             if (ts == PVVC_OK) tree_weights(a.geo, fc, wt, meta);
+            PVVC_STAMP(item, 1);
             int rs = PVVC_OK;
             if (ts == PVVC_OK) rs = rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
+            PVVC_STAMP(item, 2);
             st = ts != PVVC_OK ? ts : rs;
         } else {
             if (wave == 0) {
@@ -464,32 +717,7 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
     }
     __syncthreads();
 
-    /* ---- wait for the intra slice of this patch in the same launch ---- */
-    if (st == PVVC_OK && wait) {
-        __shared__ int wait_ok;
-        if (tid == 0) {
-            uint32_t spins = 0;
-            int ok = 1;
-            while (__hip_atomic_load(&slot->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != launch_id) {
-                if (++spins > kSpinLimit) {
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-            wait_ok = ok;
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (!wait_ok) {
-            st = PVVC_ERR_TIMEOUT;
-        } else {
-            const uint32_t sn = __hip_atomic_load(&slot->n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (sn == 0) st = PVVC_ERR_EMPTY_REFERENCE;
-            else if (sn != a.n) st = PVVC_ERR_UNMATCHED_SIZE;
-        }
-    }
-
+    PVVC_STAMP(item, 3);
     /* ---- H + M: top-down inverse RAHT, centres, leaf output ---- */
     const uint32_t h = (st == PVVC_OK) ? meta.height : 0u;
     float* pp = nullptr;
@@ -520,11 +748,15 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
             float rx = hdr[3], ry = hdr[4], rz = hdr[5];
             for (uint32_t l = 0; l + 1 < h; ++l) {
                 const uint32_t cnt = meta.cnt[l], base = meta.base[l];
-                const uint32_t cbase = meta.base[l + 1];
-                const bool leaf_level = (l + 2 == h);
+                const uint32_t cbase = meta.base[l + 1], ccnt = meta.cnt[l + 1];
+                const bool leaf_level = (l + 2 == h); /* children are leaves (h == 2) */
+                const bool fused = (l + 3 == h);      /* children are the last internal level:
+                                                         decode them and their leaves here */
                 /* SubSpaceCenter: child range = range / 2, offset = child range / 2 */
                 const float sx = rx / 2.0f, sy = ry / 2.0f, sz = rz / 2.0f;
                 const float ox = sx / 2.0f, oy = sy / 2.0f, oz = sz / 2.0f;
+                const float sx2 = sx / 2.0f, sy2 = sy / 2.0f, sz2 = sz / 2.0f;
+                const float ox2 = sx2 / 2.0f, oy2 = sy2 / 2.0f, oz2 = sz2 / 2.0f;
                 float* cur = (l & 1) ? bufB : bufA;
                 float* nxt = (l & 1) ? bufA : bufB;
                 for (uint32_t j = tid; j < cnt; j += NT) {
@@ -546,79 +778,58 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
                         cy = cur[4 * wmax + j];
                         cz = cur[5 * wmax + j];
                     }
-                    /* node weights (octree.h:99 layout) */
-                    uint32_t w[16];
+                    uint32_t cw[8];
                     uint32_t rank = 0;
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const bool has = (occ >> (7 - i)) & 1u;
-                        w[8 + i] = has ? (leaf_level ? 1u : wt[cbase + f + rank]) : 0u;
+                        cw[i] = has ? (leaf_level ? 1u : wt[cbase + f + rank]) : 0u;
                         rank += has ? 1u : 0u;
                     }
-                    w[4] = w[8] + w[12];
-                    w[5] = w[9] + w[13];
-                    w[6] = w[10] + w[14];
-                    w[7] = w[11] + w[15];
-                    w[2] = w[4] + w[6];
-                    w[3] = w[5] + w[7];
-                    /* InvertHierarchicalTransform (octree_base.cc:84-100) */
-                    float cY[16], cU[16], cV[16];
-                    cY[1] = dY;
-                    cU[1] = dU;
-                    cV[1] = dV;
-                    uint32_t r = cnt + f - j; /* reversed stream index of this node's first AC */
-                    constexpr int NW[8][2] = {{0, 0}, {2, 3}, {4, 6}, {5, 7}, {8, 12}, {9, 13}, {10, 14}, {11, 15}};
-#pragma unroll
-                    for (int idx = 1; idx < 8; ++idx) {
-                        const int ia = NW[idx][0], ib = NW[idx][1];
-                        const uint32_t w0 = w[ia], w1 = w[ib];
-                        if (w0 != 0 && w1 != 0) {
-                            const float hY = coef[n - 1 - r], hU = coef[2 * n - 1 - r], hV = coef[3 * n - 1 - r];
-                            ++r;
-                            const float base_ = sqrtf(static_cast<float>(w0 + w1));
-                            const float ta = sqrtf(static_cast<float>(w0)) / base_;
-                            const float tc = sqrtf(static_cast<float>(w1)) / base_;
-                            cY[ia] = cY[idx] * ta + hY * (-tc);
-                            cY[ib] = cY[idx] * tc + hY * ta;
-                            cU[ia] = cU[idx] * ta + hU * (-tc);
-                            cU[ib] = cU[idx] * tc + hU * ta;
-                            cV[ia] = cV[idx] * ta + hV * (-tc);
-                            cV[ib] = cV[idx] * tc + hV * ta;
-                        } else {
-                            /* one side empty: the transform passes g through to
-                             * the occupied side (T = identity up to signed zeros) */
-                            cY[ia] = cY[idx];
-                            cU[ia] = cU[idx];
-                            cV[ia] = cV[idx];
-                            cY[ib] = cY[idx];
-                            cU[ib] = cU[idx];
-                            cV[ib] = cV[idx];
-                        }
-                    }
-                    /* children: g_DC, centre (SubSpaceCenter), write */
+                    float oY[8], oU[8], oV[8];
+                    node_inverse(cw, dY, dU, dV, coef, n, cnt + f - j, oY, oU, oV);
                     uint32_t k = 0;
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
-                        if ((occ >> (7 - i)) & 1u) {
-                            const uint32_t c = f + k;
-                            ++k;
-                            const float ccx = (i & 4) ? cx - ox : cx + ox;
-                            const float ccy = (i & 2) ? cy - oy : cy + oy;
-                            const float ccz = (i & 1) ? cz - oz : cz + oz;
-                            if (leaf_level) {
-                                emit_leaf(c, ccx, ccy, ccz, cY[8 + i], cU[8 + i], cV[8 + i], a, ref, rs, predict, sink);
-                            } else {
-                                nxt[c] = cY[8 + i];
-                                nxt[wmax + c] = cU[8 + i];
-                                nxt[2 * wmax + c] = cV[8 + i];
-                                nxt[3 * wmax + c] = ccx;
-                                nxt[4 * wmax + c] = ccy;
-                                nxt[5 * wmax + c] = ccz;
+                        if (!((occ >> (7 - i)) & 1u)) continue;
+                        const uint32_t c = f + k;
+                        ++k;
+                        const float ccx = (i & 4) ? cx - ox : cx + ox;
+                        const float ccy = (i & 2) ? cy - oy : cy + oy;
+                        const float ccz = (i & 1) ? cz - oz : cz + oz;
+                        if (leaf_level) {
+                            emit_leaf(c, ccx, ccy, ccz, oY[i], oU[i], oV[i], a, ref, rs, predict, sink);
+                        } else if (fused) {
+                            /* child c on level h-2: its children are leaves of weight 1 */
+                            const uint32_t cocc = a.geo[25 + cbase + c];
+                            const uint32_t cf = fc[cbase + c];
+                            uint32_t lw[8];
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) lw[q] = (cocc >> (7 - q)) & 1u;
+                            float lY[8], lU[8], lV[8];
+                            node_inverse(lw, oY[i], oU[i], oV[i], coef, n, ccnt + cf - c, lY, lU, lV);
+                            uint32_t m = 0;
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) {
+                                if (!lw[q]) continue;
+                                const float lx = (q & 4) ? ccx - ox2 : ccx + ox2;
+                                const float ly = (q & 2) ? ccy - oy2 : ccy + oy2;
+                                const float lz = (q & 1) ? ccz - oz2 : ccz + oz2;
+                                emit_leaf(cf + m, lx, ly, lz, lY[q], lU[q], lV[q], a, ref, rs, predict, sink);
+                                ++m;
                             }
+                        } else {
+                            nxt[c] = oY[i];
+                            nxt[wmax + c] = oU[i];
+                            nxt[2 * wmax + c] = oV[i];
+                            nxt[3 * wmax + c] = ccx;
+                            nxt[4 * wmax + c] = ccy;
+                            nxt[5 * wmax + c] = ccz;
                         }
                     }
                 }
                 __syncthreads();
+                if (fused) break;
                 rx = sx;
                 ry = sy;
                 rz = sz;
@@ -626,23 +837,24 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
         }
     }
 
+    PVVC_STAMP(item, 4);
     /* ---- intra: keep geometry for the GoP, publish ---- */
     __syncthreads();
     if (!predict && (a.type & PVVC_TYPE_VALID)) {
         if (st == PVVC_OK) {
             for (uint32_t i = tid; i < a.geo_size; i += NT) slot->geo[i] = a.geo[i];
         }
-        __threadfence();
+        /* read by later launches only (kernel boundaries order them) */
         __syncthreads();
         if (tid == 0) {
             slot->n = st == PVVC_OK ? a.n : 0u;
             slot->geo_size = st == PVVC_OK ? a.geo_size : 0u;
             slot->status = st;
-            __hip_atomic_store(&slot->ready, launch_id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     } else if (tid == 0) {
         slot->status = st;
     }
+    PVVC_STAMP(item, 5);
     return st;
 }
 
@@ -683,13 +895,14 @@ __global__ __launch_bounds__(64 * WAVES) void dropin_kernel(Points* out, const i
     a.type = type[s];
     a.qp = qp[s];
     a.wait = 0;
+    a.stage = nullptr;
     a.out_off = static_cast<uint64_t>(static_cast<int64_t>(inner_offset[s]));
     const float* m = mv[s];
 #pragma unroll
     for (int i = 0; i < 12; ++i) a.mv[i] = m[i];
     Sink sink{nullptr, nullptr, out, PVVC_OUT_POINTS, white_small};
     Arena ar{g_smem, lds_bytes, 0, slot->ws, slot->ws_cap, 0};
-    int st = decode_slice<WAVES>(a, slot, ar, sink, meta, 0u, false);
+    int st = decode_slice<WAVES>(a, slot, ar, sink, meta, 0u, false, static_cast<uint32_t>(s));
     if (st != PVVC_OK && threadIdx.x == 0) atomicOr(err_flag, 2);
 }
 
@@ -704,6 +917,7 @@ __device__ __forceinline__ SliceArgs item_args(const ItemDev* __restrict__ it) {
     a.qp = it->qp;
     a.wait = it->wait;
     a.out_off = it->out_off;
+    a.stage = nullptr;
 #pragma unroll
     for (int i = 0; i < 12; ++i) a.mv[i] = it->mv[i];
     return a;
@@ -720,7 +934,7 @@ __global__ __launch_bounds__(64 * WAVES) void frame_kernel(const ItemDev* __rest
     const ItemDev* it = items + i;
     SlotDev* slot = slots + it->patch;
     Arena ar{g_smem, lds_bytes, 0, slot->ws, slot->ws_cap, 0};
-    int st = decode_slice<WAVES>(item_args(it), slot, ar, sink, meta, 0u, false);
+    int st = decode_slice<WAVES>(item_args(it), slot, ar, sink, meta, 0u, false, i);
     if (threadIdx.x == 0) status[i] = st;
 }
 
@@ -730,7 +944,8 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
                                                               const uint32_t* __restrict__ order, uint32_t count,
                                                               uint32_t* queue, SlotDev* slots, Sink sink,
                                                               uint32_t lds_bytes, uint8_t* slab, uint64_t slab_bytes,
-                                                              uint32_t launch_id, int* __restrict__ status) {
+                                                              uint32_t launch_id, int* __restrict__ status,
+                                                              float* stage) {
     __shared__ LevelMeta meta;
     __shared__ uint32_t next;
     uint8_t* ws = slab + static_cast<uint64_t>(blockIdx.x) * slab_bytes;
@@ -744,13 +959,57 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
         const ItemDev* it = items + i;
         SlotDev* slot = slots + it->patch;
         Arena ar{g_smem, lds_bytes, 0, ws, slab_bytes, 0};
-        int st = decode_slice<WAVES>(item_args(it), slot, ar, sink, meta, launch_id, true);
+        SliceArgs a = item_args(it);
+        a.stage = stage;
+        int st = decode_slice<WAVES>(a, slot, ar, sink, meta, launch_id, true, i);
         if (threadIdx.x == 0) status[i] = st;
         __syncthreads();
     }
 }
 
 __global__ void reset_counter(uint32_t* q) { *q = 0; }
+
+/* Finish pass of deferred predictive items: the intra slice of the same
+ * patch decoded in the launch before this one; add its colours
+ * (invert_raht_octree.cc:304-308), convert (common.h:154-158) and write. */
+__global__ __launch_bounds__(256) void finish_kernel(const ItemDev* __restrict__ items,
+                                                     const uint32_t* __restrict__ list, uint32_t count,
+                                                     const SlotDev* __restrict__ slots, Sink sink,
+                                                     const float* __restrict__ stage, int* __restrict__ status) {
+    if (blockIdx.x >= count) return;
+    const uint32_t i = list[blockIdx.x];
+    const ItemDev* it = items + i;
+    const SlotDev* slot = slots + it->patch;
+    const uint32_t n = it->n;
+    const uint32_t sn = slot->n;
+    int st = status[i];
+    if (sn == 0) st = PVVC_ERR_EMPTY_REFERENCE; /* the intra slice failed */
+    else if (st == PVVC_OK && sn != n) st = PVVC_ERR_UNMATCHED_SIZE;
+    if (st != PVVC_OK) {
+        if (threadIdx.x == 0) status[i] = st;
+        return;
+    }
+    const float* ref = slot->ref;
+    const uint32_t rs = slot->n_cap;
+    for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+        const uint64_t o = it->out_off + c;
+        const float* sp = stage + 3 * o;
+        const float Y = sp[0] + ref[c];
+        const float U = sp[1] + ref[rs + c];
+        const float V = sp[2] + ref[2 * rs + c];
+        const float r = Y + 1.4020f * (V - 128.0f);
+        const float g = Y - 0.3441f * (U - 128.0f) - 0.7141f * (V - 128.0f);
+        const float b = Y + 1.7720f * (U - 128.0f);
+        if (sink.mode == PVVC_OUT_NATIVE) {
+            uint8_t* pc = sink.rgb + 3 * o;
+            pc[0] = to_u8(r);
+            pc[1] = to_u8(g);
+            pc[2] = to_u8(b);
+        } else {
+            write_points_rgb(sink.pts + o, r, g, b, n, sink.white_small);
+        }
+    }
+}
 
 }  // namespace pvvc
 
@@ -821,7 +1080,7 @@ Timing g_timing;
 
 struct TuningState {
     int waves = 1;
-    int lds_bytes = 48 * 1024;
+    int lds_bytes = 24 * 1024;
     int wgs_per_cu = 8;
     int queue_chunk = 1;
 };
@@ -859,7 +1118,7 @@ uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 uint32_t geo_cap_for(int max_points) { return static_cast<uint32_t>(64 + 2ull * max_points + 25); }
 uint64_t ws_cap_for(int max_points) {
     uint64_t n = static_cast<uint64_t>(max_points);
-    return align256(12 * n + 2 * (4 * (geo_cap_for(max_points) + 1ull) + 16) + 48 * n + 64);
+    return align256(12 * n + 2 * (4 * (geo_cap_for(max_points) + 1ull) + 16) + 48 * n + 512);
 }
 
 bool state_create(int n_patches, int max_points, StateInfo& si) {
@@ -915,16 +1174,19 @@ bool state_reset(StateInfo& si) {
 /* ---- native decoder / batch ---- */
 struct pvvc_decoder {
     StateInfo st;
-    pvvc_tuning tuning{1, 48 * 1024, 8, 1};
+    pvvc_tuning tuning{1, 16 * 1024, 8, 1};
     uint32_t launch_id = 0;
     uint8_t* slab = nullptr;
     uint64_t slab_bytes = 0, slab_total = 0;
     uint32_t* queue = nullptr;
+    float* stage = nullptr;      /* unreferenced colours of deferred predictive items */
+    uint64_t stage_bytes = 0;
 };
 
 struct Segment {
-    uint32_t begin, count; /* range in order[] */
-    uint64_t ws_need;      /* per-item workspace bound */
+    uint32_t begin, count;       /* range in order[] */
+    uint32_t dbegin, dcount;     /* deferred predictive items, range in defer[] */
+    uint64_t ws_need;            /* per-item workspace bound */
 };
 
 struct pvvc_batch {
@@ -939,6 +1201,7 @@ struct pvvc_batch {
     uint8_t* dev = nullptr;
     ItemDev* items = nullptr;
     uint32_t* order = nullptr;
+    uint32_t* defer = nullptr;
     int* status = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_output = -1;
@@ -961,7 +1224,10 @@ void launch_persistent(pvvc_batch* b, const Segment& sg, const Sink& sink, uint3
     pvvc_decoder* d = b->dec;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, d->queue);
     hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b->items, b->order + sg.begin, sg.count,
-                       d->queue, d->st.slots, sink, lds, d->slab, d->slab_bytes, launch_id, b->status);
+                       d->queue, d->st.slots, sink, lds, d->slab, d->slab_bytes, launch_id, b->status, d->stage);
+    if (sg.dcount)
+        hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, b->items, b->defer + sg.dbegin, sg.dcount,
+                           d->st.slots, sink, d->stage, b->status);
 }
 
 template <int W>
@@ -982,6 +1248,13 @@ void set_all_lds_attrs(int bytes) {
 extern "C" {
 
 int pvvc_last_error(void) { return g_last_error.exchange(0); }
+
+#ifdef PVVC_PHASE_PROFILE
+/* Profiling build only: installs the stamp buffer (device, 8 u64 per item). */
+int pvvc_phase_profile_set(void* dev_buf) {
+    return hip_ok(hipMemcpyToSymbol(HIP_SYMBOL(g_prof), &dev_buf, sizeof(void*))) ? PVVC_OK : PVVC_ERR_HIP;
+}
+#endif
 
 int pvvc_set_stream(void* stream) {
     g_stream = static_cast<hipStream_t>(stream);
@@ -1217,6 +1490,7 @@ int pvvc_decoder_destroy(pvvc_decoder* d) {
     state_destroy(d->st);
     if (d->slab) hipFree(d->slab);
     if (d->queue) hipFree(d->queue);
+    if (d->stage) hipFree(d->stage);
     delete d;
     return PVVC_OK;
 }
@@ -1266,7 +1540,8 @@ int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_fr
     b->n_items = n_items;
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t o_items = 0, o_order = al(sizeof(ItemDev) * n_items), o_status = al(o_order + 4ull * n_items);
-    const size_t o_data = al(o_status + 4ull * n_items);
+    const size_t o_defer = al(o_status + 4ull * n_items);
+    const size_t o_data = al(o_defer + 4ull * n_items);
     const size_t total = o_data + data_bytes;
     if (!hip_ok(hipMalloc(&b->dev, std::max<size_t>(total, 256)))) {
         delete b;
@@ -1351,7 +1626,7 @@ int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_fr
     (void)split_needed;
     b->total_points = pts_acc;
     /* queue order per segment: longest first; intra before predictive of equal length */
-    std::vector<uint32_t> order(n_items);
+    std::vector<uint32_t> order(n_items), defer;
     seg_starts.push_back(n_items);
     for (size_t sidx = 0; sidx + 1 < seg_starts.size(); ++sidx) {
         uint32_t lo = seg_starts[sidx], hi = seg_starts[sidx + 1];
@@ -1373,14 +1648,19 @@ int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_fr
         for (uint32_t i : ids) {
             const ItemDev& x = items[i];
             uint64_t g = x.geo_size ? x.geo_size : d->st.geo_cap;
-            uint64_t bound = 12ull * x.n + 2 * (4 * (g + 1) + 16) + 48ull * x.n + 64;
+            uint64_t bound = 12ull * x.n + 2 * (4 * (g + 1) + 16) + 48ull * x.n + 512;
             need = std::max(need, bound);
         }
         std::copy(ids.begin(), ids.end(), order.begin() + lo);
-        b->segs.push_back(Segment{lo, hi - lo, (need + 255) & ~uint64_t(255)});
+        const uint32_t dlo = static_cast<uint32_t>(defer.size());
+        for (uint32_t i = lo; i < hi; ++i)
+            if (items[i].wait) defer.push_back(i);
+        b->segs.push_back(Segment{lo, hi - lo, dlo, static_cast<uint32_t>(defer.size()) - dlo,
+                                  (need + 255) & ~uint64_t(255)});
     }
     std::memcpy(&h[o_items], items.data(), sizeof(ItemDev) * n_items);
     std::memcpy(&h[o_order], order.data(), 4ull * n_items);
+    if (!defer.empty()) std::memcpy(&h[o_defer], defer.data(), 4ull * defer.size());
     if (!hip_ok(hipMemcpy(b->dev, h.data(), total, hipMemcpyHostToDevice))) {
         hipFree(b->dev);
         delete b;
@@ -1388,6 +1668,7 @@ int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_fr
     }
     b->items = reinterpret_cast<ItemDev*>(b->dev + o_items);
     b->order = reinterpret_cast<uint32_t*>(b->dev + o_order);
+    b->defer = reinterpret_cast<uint32_t*>(b->dev + o_defer);
     b->status = reinterpret_cast<int*>(b->dev + o_status);
     *out = b;
     return PVVC_OK;
@@ -1489,6 +1770,19 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
         }
         d->slab_total = need * grid_max;
         d->slab_bytes = need;
+    }
+    const uint64_t stage_need = 12ull * static_cast<uint64_t>(std::max<int64_t>(b->total_points, 1));
+    if (stage_need > d->stage_bytes) {
+        if (d->stage) {
+            hipStreamSynchronize(g_stream);
+            hipFree(d->stage);
+            d->stage = nullptr;
+        }
+        if (!hip_ok(hipMalloc(&d->stage, stage_need))) {
+            d->stage_bytes = 0;
+            return PVVC_ERR_HIP;
+        }
+        d->stage_bytes = stage_need;
     }
     for (auto& sg : b->segs) {
         if (!sg.count) continue;

@@ -28,8 +28,12 @@
  *      (common.h:154-158), written straight to the output.
  *
  * Intra slices keep their geometry bytes and decoded colours in the patch's
- * state slot for the predictive slices of the GoP.  Workspace arrays live
- * in LDS when they fit the per-item LDS budget, else in a global slab.
+ * state slot for the predictive slices of the GoP.  A predictive slice whose
+ * intra slice is decoded in the same launch does not wait for it: it writes
+ * its unreferenced leaf colours to a staging buffer and a finish pass after
+ * the launch adds the reference (no cross-workgroup synchronisation, no
+ * device-wide fences inside a launch).  Workspace arrays live in LDS when
+ * they fit the per-item LDS budget, else in a global slab.
  *
  * Arithmetic is compiled with -ffp-contract=off and correctly rounded
  * divide/sqrt so that FP32 results equal the reference CPU decoder's.
@@ -45,8 +49,6 @@
 namespace pvvc {
 
 constexpr int kMaxHeight = 64;
-/* Polls of the intra-ready flag before a predictive item gives up. */
-constexpr uint32_t kSpinLimit = 1u << 24;
 
 /* Per-patch state slot (device).  `ws` is this patch's private workspace
  * for launches that decode each patch at most once. */
@@ -60,7 +62,7 @@ struct SlotDev {
     uint32_t n;        /* points of the current GoP, 0 = no intra slice */
     uint32_t geo_size;
     int32_t status;    /* status of the last slice decoded for this patch */
-    uint32_t ready;    /* id of the launch whose intra slice last completed */
+    uint32_t ready;    /* reserved */
     uint32_t pad[2];
 };
 
@@ -75,7 +77,8 @@ struct ItemDev {
     int32_t patch;
     uint8_t type;
     uint8_t qp;
-    uint8_t wait;         /* predictive item whose intra item is in the same batch */
+    uint8_t wait;         /* predictive item whose intra item is in the same batch:
+                             its reference colours are added after the launch */
     uint8_t pad0;
     uint32_t pad1;
     uint64_t out_off;     /* first output point */
@@ -103,6 +106,10 @@ struct SliceArgs {
     uint32_t wait;
     uint64_t out_off;
     float mv[12];
+    float* stage;      /* predictive slice whose intra slice is decoded in the same
+                          launch: unreferenced leaf YUV goes here (3 floats at
+                          3 * (out_off + leaf)); the reference is added by the
+                          finish pass after the launch */
 };
 
 /* Level metadata shared through LDS. */

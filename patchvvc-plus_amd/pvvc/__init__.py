@@ -21,6 +21,7 @@ from . import _native
 from ._native import DropinFrame, Points, SynthConfig, Tuning, enc_lib, hip_lib
 
 __all__ = [
+    "PvvcError", "StagedFrame", "launch_dropin", "synchronize", "last_kernel_ms", "last_launch_times",
     "STATUS", "Frame", "DeviceBuffer", "State", "Decoder", "Batch", "decode_dropin", "synth_stream",
     "SYNTH_PRESETS", "Points", "hip_lib", "enc_lib", "MODE_PER_FRAME", "MODE_PERSISTENT", "MODE_GRAPH",
     "OUT_NATIVE", "OUT_POINTS", "device_info",
