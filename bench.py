@@ -39,6 +39,83 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else float("nan")
 
 
+def _pmc_kib(csv_path: Path, kernel: str) -> float | None:
+    """Mean counter value (KiB per dispatch) of the dispatches of `kernel`."""
+    import csv
+    vals = []
+    with open(csv_path, newline="") as fh:
+        for row in csv.DictReader(fh):
+            if kernel in row["Kernel_Name"]:
+                vals.append(float(row["Counter_Value"]))
+    return float(np.mean(vals)) if vals else None
+
+
+def measure_traffic(args) -> dict | None:
+    """HBM bytes per persistent_kernel launch from two rocprofv3 --pmc passes
+    (FETCH_SIZE, then WRITE_SIZE) of this script's batched leg, each scaled by
+    what the same counter reads for a device copy of 256 MiB.  Runs as child
+    processes before this process touches the GPU; None when rocprofv3 is
+    absent or a pass fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    me = [sys.executable, "-u", str(Path(__file__).resolve())]
+    leg = me + ["--only", "batched", "--steps", "2", "--warmup", "1", "--no-cpu", "--no-traffic",
+                "--preset", args.preset, "--seed", str(args.seed)]
+    for k in ("waves", "lds", "wgs_per_cu", "queue_chunk"):
+        v = getattr(args, k)
+        if v:
+            leg += ["--" + k.replace("_", "-"), str(v)]
+    copy_mib = 256
+    res = {}
+    with tempfile.TemporaryDirectory() as d:
+        for tag, counter, cmd, kern in (("fetch", "FETCH_SIZE", leg, "persistent_kernel"),
+                                         ("write", "WRITE_SIZE", leg, "persistent_kernel"),
+                                         ("copy_fetch", "FETCH_SIZE", me + ["--calibrate-copy", str(copy_mib)],
+                                          "copyBuffer"),
+                                         ("copy_write", "WRITE_SIZE", me + ["--calibrate-copy", str(copy_mib)],
+                                          "copyBuffer")):
+            out = Path(d) / tag
+            full = [prof, "--pmc", counter, "-f", "csv", "-d", str(out), "-o", "p", "--"] + cmd
+            log("traffic pass:", tag, counter)
+            try:
+                r = subprocess.run(full, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120)
+            except subprocess.TimeoutExpired:
+                log("traffic pass timed out:", tag)
+                return None
+            csvs = list(out.rglob("*counter_collection.csv"))
+            if r.returncode != 0 or not csvs:
+                log("traffic pass failed:", tag, r.returncode, r.stdout[-2000:].decode(errors="replace"))
+                return None
+            v = _pmc_kib(csvs[0], kern)
+            if v is None:
+                return None
+            res[tag] = v
+    copy_kib = copy_mib * 1024.0
+    f_scale = copy_kib / res["copy_fetch"]
+    w_scale = copy_kib / res["copy_write"]
+    fetch_b = res["fetch"] * f_scale * 1024.0
+    write_b = res["write"] * w_scale * 1024.0
+    return {"bytes": fetch_b + write_b, "fetch_bytes": fetch_b, "write_bytes": write_b,
+            "fetch_kib_raw": res["fetch"], "write_kib_raw": res["write"],
+            "fetch_scale": f_scale, "write_scale": w_scale,
+            "calibration": f"device copy of {copy_mib} MiB reads FETCH_SIZE {res['copy_fetch']:.1f} KiB, "
+                           f"WRITE_SIZE {res['copy_write']:.1f} KiB"}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,13 +129,35 @@ def main() -> int:
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--lds", type=int, default=0)
     ap.add_argument("--wgs-per-cu", type=int, default=0)
+    ap.add_argument("--queue-chunk", type=int, default=0)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
+    ap.add_argument("--only", choices=["all", "batched", "perframe"], default="all",
+                    help="restrict the GPU legs (profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU path is design-only "
               "(patches shard by stream / patch-index range with no exchange).", file=sys.stderr)
         return 2
 
+    traffic = None
+    if not args.no_traffic and not args.calibrate_copy and args.only == "all":
+        traffic = measure_traffic(args)
+        log("traffic", traffic)
+
     import pvvc
+
+    if args.calibrate_copy:
+        import ctypes as C
+        nbytes = args.calibrate_copy << 20
+        src, dst = pvvc.DeviceBuffer(nbytes), pvvc.DeviceBuffer(nbytes)
+        src.zero()
+        for _ in range(3):
+            pvvc.hip_lib().pvvc_memcpy_d2d(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes)
+        pvvc.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 3}))
+        return 0
 
     info = pvvc.device_info()
     log("device", info)
@@ -73,37 +172,54 @@ def main() -> int:
     pts_per_frame = [f.points for f in frames]
 
     # ---------------- batched persistent decode (the headline) ----------------
+    run_b = args.only in ("all", "batched")
+    run_p = args.only in ("all", "perframe")
     dec = pvvc.Decoder(n_patches, max_pts)
-    tun = {k: v for k, v in dict(waves_per_patch=args.waves, lds_bytes=args.lds, wgs_per_cu=args.wgs_per_cu).items() if v}
+    tun = {k: v for k, v in dict(waves_per_patch=args.waves, lds_bytes=args.lds, wgs_per_cu=args.wgs_per_cu,
+                                 queue_chunk=args.queue_chunk).items() if v}
     if tun:
         dec.set_tuning(**tun)
     batches = [dec.batch(frames[g * gop:(g + 1) * gop]) for g in range(args.gops)]
     outs = [dec.outputs(b, pvvc.OUT_NATIVE) for b in batches]
-    for i in range(args.warmup):
+    for i in range(args.warmup if run_b else 0):
         dec.decode(batches[i % args.gops], pvvc.MODE_PERSISTENT, outs[i % args.gops])
     pvvc.synchronize()
     bad = [int((b.status() != 0).sum()) for b in batches]
     if any(bad):
         log("slice errors in warmup:", bad)
-    kernel_ms, wall = [], []
-    for i in range(args.steps):
+    kernel_ms, wall, pk_ms, pk_launches = [], [], [], 0
+    for i in range(args.steps if run_b else 0):
         b = batches[i % args.gops]
         ts = time.perf_counter()
         dec.decode(b, pvvc.MODE_PERSISTENT, outs[i % args.gops])
         pvvc.synchronize()
         wall.append((time.perf_counter() - ts) * 1e3)
         kernel_ms.append(pvvc.last_kernel_ms())
+        kt = pvvc.last_decode_kernel_times()
+        pk_ms.append(sum(kt))
+        pk_launches += len(kt)
     step_pts = [b.total_points for b in batches]
     tot_pts = sum(step_pts[i % args.gops] for i in range(args.steps))
-    tot_ms = sum(wall)
+    tot_ms = sum(wall) if wall else float("nan")
+    kernel_ms = kernel_ms or [float("nan")]
     batched_mpts = tot_pts / (tot_ms * 1e-3) / 1e6
     batched_fps = args.steps * gop / (tot_ms * 1e-3)
     in_bytes = [b.input_bytes for b in batches]
     alg_bytes = [in_bytes[g] + B_OUT_NATIVE * step_pts[g] for g in range(args.gops)]
     avg_kernel_ms = float(np.mean(kernel_ms))
-    avg_alg = float(np.mean([alg_bytes[i % args.gops] for i in range(args.steps)]))
-    achieved_gbs = avg_alg / (avg_kernel_ms * 1e-3) / 1e9
-    log(f"persistent: {avg_kernel_ms:.3f} ms/GoP kernel, {np.mean(wall):.3f} ms wall, {batched_mpts:.1f} Mpts/s")
+    # the dominant kernel: persistent_kernel launches alone, from their own events
+    n_launch = max(1, pk_launches)
+    pk_avg_ms = float(np.sum(pk_ms)) / n_launch if pk_ms else float("nan")
+    alg_per_launch = float(np.sum([alg_bytes[i % args.gops] for i in range(args.steps)])) / n_launch
+    achieved_gbs = alg_per_launch / (pk_avg_ms * 1e-3) / 1e9
+    # resident GoP state read by predictive slices: intra geometry + reference YUV (12 B/pt)
+    state_bytes = [b.state_bytes for b in batches]
+    sym_per_step = [3 * p for p in step_pts]
+    cu = int(info["cu"])
+    tot_sym = sum(sym_per_step[i % args.gops] for i in range(args.steps))
+    sym_per_s_cu = tot_sym / (float(np.sum(pk_ms)) * 1e-3) / cu if pk_ms else float("nan")
+    log(f"persistent: {avg_kernel_ms:.3f} ms/GoP all launches, {pk_avg_ms:.3f} ms per persistent_kernel, "
+        f"{np.mean(wall):.3f} ms wall, {batched_mpts:.1f} Mpts/s")
 
     # ---------------- per-frame drop-in decode (launch_cudaProcess) ----------------
     state = pvvc.State(n_patches, max_pts)
@@ -111,10 +227,10 @@ def main() -> int:
     out_pts = pvvc.DeviceBuffer(24 * max(pts_per_frame))
     lat = []
     frame_kernel_ms = []
-    for i in range(min(len(frames), gop) * 1):  # warm: one GoP
+    for i in range(min(len(frames), gop) if run_p else 0):  # warm: one GoP
         pvvc.launch_dropin(staged[i], state, out_pts)
     pvvc.synchronize()
-    n_pf = args.steps * gop
+    n_pf = args.steps * gop if run_p else 0
     t_pf0 = time.perf_counter()
     for i in range(n_pf):
         k = i % len(frames)
@@ -125,13 +241,14 @@ def main() -> int:
         frame_kernel_ms.append(pvvc.last_kernel_ms())
     pf_ms = (time.perf_counter() - t_pf0) * 1e3
     pf_pts = sum(pts_per_frame[i % len(frames)] for i in range(n_pf))
-    pf_fps = n_pf / (pf_ms * 1e-3)
-    pf_mpts = pf_pts / (pf_ms * 1e-3) / 1e6
+    pf_fps = n_pf / (pf_ms * 1e-3) if n_pf else float("nan")
+    pf_mpts = pf_pts / (pf_ms * 1e-3) / 1e6 if n_pf else float("nan")
+    frame_kernel_ms = frame_kernel_ms or [float("nan")]
     log(f"per-frame drop-in: {np.mean(frame_kernel_ms):.3f} ms kernel, p50 {pct(lat, 50):.3f} p99 {pct(lat, 99):.3f} ms,"
         f" {pf_fps:.1f} fps")
 
     # ---------------- CPU baseline: oracle on a bounded sample ----------------
-    cpu = None
+    cpu, accuracy = None, None
     if not args.no_cpu:
         import tempfile
 
@@ -140,25 +257,55 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         od = binding.StreamDecoder(fp64=False)
         with tempfile.TemporaryDirectory() as d:
-            done_pts, done_ms, nfr = 0, 0.0, 0
+            done_pts, done_ms, done_ms_ref, nfr = 0, 0.0, 0.0, 0
+            frame_ms = []
             bufs = {"xyz": np.zeros((max(pts_per_frame), 3), np.float32),
                     "rgb": np.zeros((max(pts_per_frame), 3), np.uint8),
                     "status": np.zeros(max(f.slice_cnt for f in frames), np.int32)}
+            paths = []
+            for k, img in enumerate(images):
+                p = Path(d) / f"cpu_{k}.frame"
+                p.write_bytes(img)
+                paths.append(p)
             t_start = time.time()
             while time.time() - t_start < args.cpu_seconds and nfr < 4 * len(frames):
                 k = nfr % len(frames)
-                p = Path(d) / f"cpu_{k}.frame"
-                if not p.exists():
-                    p.write_bytes(images[k])
-                of = binding.OracleFrame(str(p))
+                of = binding.OracleFrame(str(paths[k]))
                 ms_ref, ms_all = od.decode_timing_only(of, threads, bufs)
                 done_ms += ms_all
+                done_ms_ref += ms_ref
+                frame_ms.append(ms_all)
                 done_pts += pts_per_frame[k]
                 nfr += 1
-        cpu = {"value": done_pts / (done_ms * 1e-3) / 1e6, "unit": "Mpts/s", "cores": threads, "kind": "port",
-               "sample": f"{nfr} frames ({done_pts} points) of the same stream, in decode order, "
-                         f"SetSlice+GetPatch+motion wall time, {threads} threads"}
-        log("cpu baseline", cpu)
+            cpu = {"value": round(done_pts / (done_ms * 1e-3) / 1e6, 3), "unit": "Mpts/s", "cores": threads,
+                   "kind": "port",
+                   "sample": f"{nfr} frames ({done_pts} points) of the same stream in decode order; "
+                             f"SetSlice+GetPatch+motion wall time per frame, {threads} threads",
+                   "value_reference_scope": round(done_pts / (done_ms_ref * 1e-3) / 1e6, 3),
+                   "frames_per_s": round(nfr / (done_ms * 1e-3), 2),
+                   "p50_frame_ms": round(pct(frame_ms, 50), 3), "p99_frame_ms": round(pct(frame_ms, 99), 3),
+                   "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+            log("cpu baseline", cpu)
+
+            # colour accuracy of the batched GPU output against the FP64 inverse-RAHT
+            # oracle on the first two frames of the GoP (one intra, one predictive)
+            if run_b:
+                xyz_g, rgb_g = dec.decode_host(batches[0], pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+                o64 = binding.StreamDecoder(fp64=True)
+                diffs, exact, geo_ok, cnt = [], 0, True, 0
+                for k in range(min(2, gop)):
+                    ref = o64.decode(binding.OracleFrame(str(paths[k])), threads)
+                    o, n = batches[0].frame_offset[k], batches[0].frame_points[k]
+                    geo_ok &= bool(np.array_equal(xyz_g[o:o + n], ref["xyz"]))
+                    exact += int((rgb_g[o:o + n] == ref["rgb"]).all(1).sum())
+                    cnt += n
+                    diffs.append(rgb_g[o:o + n].astype(np.float64) - np.clip(ref["rgb64"], 0.0, 255.0))
+                dd = np.concatenate(diffs)
+                accuracy = {"frames": min(2, gop), "points": cnt, "geometry_bit_exact": geo_ok,
+                            "rgb8_exact_vs_fp32_oracle": round(exact / max(1, cnt), 6),
+                            "max_abs_err_vs_fp64": round(float(np.abs(dd).max()), 4),
+                            "rms_err_vs_fp64": round(float(np.sqrt((dd ** 2).mean())), 4)}
+                log("accuracy", accuracy)
 
     res = {
         "metric": "decoded Mpts/s (batched persistent GoP decode), 1 MI355X",
@@ -185,9 +332,14 @@ def main() -> int:
         "batched_vs_per_frame": round(batched_fps / pf_fps, 3),
         "p99_frame_latency_ms": round(pct(lat, 99), 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "persistent_kernel", "alg_bytes_per_launch": int(avg_alg),
-                     "kernel_ms": round(avg_kernel_ms, 4)},
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
+                     "traffic": round(traffic["bytes"]) if traffic else None,
+                     "kernel": "persistent_kernel", "alg_bytes_per_launch": int(alg_per_launch),
+                     "kernel_ms": round(pk_avg_ms, 4), "launches": pk_launches,
+                     "traffic_detail": traffic},
+        "state_bytes_per_step": int(np.mean(state_bytes)),
+        "rlgr_symbols_per_s_per_cu": round(sym_per_s_cu, 1),
+        "accuracy": accuracy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -171,7 +171,12 @@ void pvvc_batch_free(pvvc_batch* batch);
 int pvvc_batch_frames(const pvvc_batch* batch);
 int64_t pvvc_batch_frame_points(const pvvc_batch* batch, int frame);
 int64_t pvvc_batch_frame_offset(const pvvc_batch* batch, int frame);
+/* Bytes the device reads from the batch's input: 82 B of slice metadata per
+ * slice plus the unwrapped geometry and colour blobs. */
 int64_t pvvc_batch_input_bytes(const pvvc_batch* batch);
+/* Bytes of resident GoP state the batch's predictive slices read: their
+ * patch's intra geometry blob plus 12 B/pt of reference YUV. */
+int64_t pvvc_batch_state_bytes(const pvvc_batch* batch);
 int pvvc_batch_items(const pvvc_batch* batch);
 
 /* Output modes. */
@@ -201,14 +206,22 @@ void* pvvc_dev_alloc(size_t bytes);
 void pvvc_dev_free(void* ptr);
 int pvvc_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int pvvc_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy on the library stream (asynchronous). */
+int pvvc_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int pvvc_memset(void* dst, int value, size_t bytes);
 int pvvc_synchronize(void);
-/* Kernel time of the last decode call, from HIP events recorded on the
- * library stream around its launches (milliseconds). */
+/* Kernel time of the last decode call made on the calling thread, from HIP
+ * events recorded on the library stream around its launches (milliseconds;
+ * includes the queue-reset and finish launches of a persistent decode). */
 double pvvc_last_kernel_ms(void);
-/* Per-launch kernel times (ms) of the last decode call; returns the launch
- * count (fills at most `max`). */
+/* Per-launch times (ms) of the last decode call on the calling thread;
+ * returns the launch count (fills at most `max`). */
 int pvvc_last_launch_times(double* ms, int max);
+/* Times (ms) of the decode kernels alone of the last decode call on the
+ * calling thread: each persistent_kernel, frame_kernel or dropin_kernel
+ * launch, events recorded directly before and after it on its stream
+ * (none for a graph replay).  Returns the launch count. */
+int pvvc_last_decode_kernel_times(double* ms, int max);
 /* Device properties used for launch shapes: CU count, LDS per workgroup. */
 int pvvc_device_info(int* cu_count, int* lds_per_block, int* warp_size, char* name, int name_len);
 

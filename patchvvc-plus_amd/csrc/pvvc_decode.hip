@@ -945,25 +945,29 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
                                                               uint32_t* queue, SlotDev* slots, Sink sink,
                                                               uint32_t lds_bytes, uint8_t* slab, uint64_t slab_bytes,
                                                               uint32_t launch_id, int* __restrict__ status,
-                                                              float* stage) {
+                                                              float* stage, uint32_t chunk) {
     __shared__ LevelMeta meta;
     __shared__ uint32_t next;
     uint8_t* ws = slab + static_cast<uint64_t>(blockIdx.x) * slab_bytes;
     for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
+        /* one atomic per `chunk` consecutive queue entries */
+        if (threadIdx.x == 0) next = atomicAdd(queue, chunk);
         __syncthreads();
-        const uint32_t q = next;
+        const uint32_t q0 = next;
         __syncthreads();
-        if (q >= count) break;
-        const uint32_t i = order[q];
-        const ItemDev* it = items + i;
-        SlotDev* slot = slots + it->patch;
-        Arena ar{g_smem, lds_bytes, 0, ws, slab_bytes, 0};
-        SliceArgs a = item_args(it);
-        a.stage = stage;
-        int st = decode_slice<WAVES>(a, slot, ar, sink, meta, launch_id, true, i);
-        if (threadIdx.x == 0) status[i] = st;
-        __syncthreads();
+        if (q0 >= count) break;
+        const uint32_t qe = q0 + chunk < count ? q0 + chunk : count;
+        for (uint32_t q = q0; q < qe; ++q) {
+            const uint32_t i = order[q];
+            const ItemDev* it = items + i;
+            SlotDev* slot = slots + it->patch;
+            Arena ar{g_smem, lds_bytes, 0, ws, slab_bytes, 0};
+            SliceArgs a = item_args(it);
+            a.stage = stage;
+            int st = decode_slice<WAVES>(a, slot, ar, sink, meta, launch_id, true, i);
+            if (threadIdx.x == 0) status[i] = st;
+            __syncthreads();
+        }
     }
 }
 
@@ -1076,7 +1080,9 @@ struct Timing {
         return r;
     }
 };
-Timing g_timing;
+/* Per host thread: a decode call and its timing queries run on the same thread. */
+thread_local Timing g_timing;   /* whole decode call: every launch of it, in order */
+thread_local Timing g_ktiming;  /* the decode kernel launches alone (frame / persistent / drop-in) */
 
 struct TuningState {
     int waves = 1;
@@ -1181,6 +1187,8 @@ struct pvvc_decoder {
     uint32_t* queue = nullptr;
     float* stage = nullptr;      /* unreferenced colours of deferred predictive items */
     uint64_t stage_bytes = 0;
+    std::vector<uint32_t> intra_geo; /* geometry bytes of each patch's latest intra slice
+                                        seen by pvvc_batch_create (state-read accounting) */
 };
 
 struct Segment {
@@ -1198,6 +1206,7 @@ struct pvvc_batch {
     uint32_t n_items = 0;
     int64_t total_points = 0;
     int64_t input_bytes = 0;
+    int64_t state_bytes = 0;     /* resident GoP state read by predictive items */
     uint8_t* dev = nullptr;
     ItemDev* items = nullptr;
     uint32_t* order = nullptr;
@@ -1222,9 +1231,14 @@ template <int W>
 void launch_persistent(pvvc_batch* b, const Segment& sg, const Sink& sink, uint32_t lds, uint32_t grid,
                        uint32_t launch_id, hipStream_t s) {
     pvvc_decoder* d = b->dec;
+    const uint32_t chunk = static_cast<uint32_t>(std::max(1, d->tuning.queue_chunk));
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, d->queue);
+    auto ev = g_ktiming.next();
+    hipEventRecord(ev.first, s);
     hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b->items, b->order + sg.begin, sg.count,
-                       d->queue, d->st.slots, sink, lds, d->slab, d->slab_bytes, launch_id, b->status, d->stage);
+                       d->queue, d->st.slots, sink, lds, d->slab, d->slab_bytes, launch_id, b->status, d->stage,
+                       chunk);
+    hipEventRecord(ev.second, s);
     if (sg.dcount)
         hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, b->items, b->defer + sg.dbegin, sg.dcount,
                            d->st.slots, sink, d->stage, b->status);
@@ -1290,6 +1304,9 @@ int pvvc_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 int pvvc_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)) ? PVVC_OK : PVVC_ERR_HIP;
 }
+int pvvc_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream)) ? PVVC_OK : PVVC_ERR_HIP;
+}
 int pvvc_memset(void* dst, int value, size_t bytes) {
     return hip_ok(hipMemset(dst, value, bytes)) ? PVVC_OK : PVVC_ERR_HIP;
 }
@@ -1303,6 +1320,15 @@ double pvvc_last_kernel_ms(void) {
         if (hipEventElapsedTime(&ms, g_timing.ev[i], g_timing.ev[i + 1]) == hipSuccess) tot += ms;
     }
     return tot;
+}
+int pvvc_last_decode_kernel_times(double* ms, int max) {
+    int k = 0;
+    for (size_t i = 0; i + 1 < g_ktiming.used && k < max; i += 2, ++k) {
+        float t = 0;
+        hipEventElapsedTime(&t, g_ktiming.ev[i], g_ktiming.ev[i + 1]);
+        if (ms) ms[k] = t;
+    }
+    return static_cast<int>(g_ktiming.used / 2);
 }
 int pvvc_last_launch_times(double* ms, int max) {
     int k = 0;
@@ -1382,8 +1408,11 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
     static std::once_flag once;
     std::call_once(once, [] { set_all_lds_attrs(clamp_lds(1 << 30)); });
     g_timing.reset();
+    g_ktiming.reset();
     auto ev = g_timing.next();
+    auto kev = g_ktiming.next();
     hipEventRecord(ev.first, g_stream);
+    hipEventRecord(kev.first, g_stream);
     switch (g_dropin_tuning.waves) {
         case 2:
             hipLaunchKernelGGL(dropin_kernel<2>, dim3(patch_size), dim3(128), lds, g_stream, cudaData, inner_offset,
@@ -1401,6 +1430,7 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
                                patch_size, static_cast<uint32_t>(lds), g_white_small, si.err);
     }
     hip_ok(hipGetLastError());
+    hipEventRecord(kev.second, g_stream);
     hipEventRecord(ev.second, g_stream);
 }
 
@@ -1480,6 +1510,7 @@ int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_
         delete d;
         return PVVC_ERR_HIP;
     }
+    d->intra_geo.assign(static_cast<size_t>(max_patches), 0u);
     *out = d;
     return PVVC_OK;
 }
@@ -1600,6 +1631,10 @@ int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_fr
                 delete b;
                 return PVVC_ERR_INVALID_ARG;
             }
+            if (x.type & PVVC_TYPE_PREDICT)
+                b->state_bytes += static_cast<int64_t>(d->intra_geo[p]) + 12ll * x.n;
+            else
+                d->intra_geo[p] = x.geo_size;
             if (x.type & PVVC_TYPE_PREDICT) {
                 auto ii = intra_item.find(p);
                 if (ii != intra_item.end()) {
@@ -1690,6 +1725,7 @@ int64_t pvvc_batch_frame_offset(const pvvc_batch* b, int f) {
     return (b && f >= 0 && f < b->n_frames) ? b->frame_off[f] : -1;
 }
 int64_t pvvc_batch_input_bytes(const pvvc_batch* b) { return b ? b->input_bytes : -1; }
+int64_t pvvc_batch_state_bytes(const pvvc_batch* b) { return b ? b->state_bytes : -1; }
 int pvvc_batch_items(const pvvc_batch* b) { return b ? static_cast<int>(b->n_items) : 0; }
 
 int pvvc_batch_status(pvvc_batch* b, int* status, int n) {
@@ -1710,6 +1746,7 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
     const int W = d->tuning.waves_per_patch;
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(d->tuning.lds_bytes));
     g_timing.reset();
+    g_ktiming.reset();
     if (mode == PVVC_MODE_PER_FRAME || mode == PVVC_MODE_GRAPH) {
         if (mode == PVVC_MODE_GRAPH) {
             void* ptrs[3] = {xyz, rgb, points};
@@ -1740,11 +1777,15 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
             return PVVC_OK;
         }
         for (int f = 0; f < b->n_frames; ++f) {
+            if (!b->frame_count[f]) continue;
             auto ev = g_timing.next();
+            auto kev = g_ktiming.next();
             hipEventRecord(ev.first, g_stream);
+            hipEventRecord(kev.first, g_stream);
             if (W == 2) launch_frame<2>(b, f, sink, lds, g_stream);
             else if (W == 4) launch_frame<4>(b, f, sink, lds, g_stream);
             else launch_frame<1>(b, f, sink, lds, g_stream);
+            hipEventRecord(kev.second, g_stream);
             hipEventRecord(ev.second, g_stream);
         }
         return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;

@@ -183,6 +183,15 @@ def last_launch_times() -> list[float]:
     return a[:n].tolist()
 
 
+def last_decode_kernel_times() -> list[float]:
+    """Durations (ms) of the decode kernels alone in the last decode call."""
+    L = hip_lib()
+    n = L.pvvc_last_decode_kernel_times(None, 0)
+    a = np.zeros(max(n, 1), np.float64)
+    L.pvvc_last_decode_kernel_times(a.ctypes.data, n)
+    return a[:n].tolist()
+
+
 # ---------------------------------------------------------------------------
 # Drop-in path
 # ---------------------------------------------------------------------------
@@ -269,6 +278,7 @@ class Batch:
         self.frame_offset = [int(L.pvvc_batch_frame_offset(h, f)) for f in range(self.n_frames)]
         self.total_points = int(sum(self.frame_points))
         self.input_bytes = int(L.pvvc_batch_input_bytes(h))
+        self.state_bytes = int(L.pvvc_batch_state_bytes(h))
         self.items = int(L.pvvc_batch_items(h))
 
     def free(self) -> None:

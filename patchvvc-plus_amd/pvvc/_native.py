@@ -94,6 +94,7 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_batch_frame_points", C.c_int64, c_vp, C.c_int)
     _sig(L, "pvvc_batch_frame_offset", C.c_int64, c_vp, C.c_int)
     _sig(L, "pvvc_batch_input_bytes", C.c_int64, c_vp)
+    _sig(L, "pvvc_batch_state_bytes", C.c_int64, c_vp)
     _sig(L, "pvvc_batch_status", C.c_int, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_decode_batch", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp)
     _sig(L, "pvvc_dev_alloc", c_vp, C.c_size_t)
@@ -101,9 +102,11 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_memcpy_h2d", C.c_int, c_vp, c_vp, C.c_size_t)
     _sig(L, "pvvc_memcpy_d2h", C.c_int, c_vp, c_vp, C.c_size_t)
     _sig(L, "pvvc_memset", C.c_int, c_vp, C.c_int, C.c_size_t)
+    _sig(L, "pvvc_memcpy_d2d", C.c_int, c_vp, c_vp, C.c_size_t)
     _sig(L, "pvvc_synchronize", C.c_int)
     _sig(L, "pvvc_last_kernel_ms", C.c_double)
     _sig(L, "pvvc_last_launch_times", C.c_int, c_vp, C.c_int)
+    _sig(L, "pvvc_last_decode_kernel_times", C.c_int, c_vp, C.c_int)
     _sig(L, "pvvc_device_info", C.c_int, c_i32p, c_i32p, c_i32p, C.c_char_p, C.c_int)
     _hip = L
     return L
