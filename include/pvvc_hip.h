@@ -2,7 +2,7 @@
  * pvvc_hip.h -- C ABI of the MI355X-native patchVVC per-patch GoP decoder.
  *
  * Drop-in surface: the reference client's decode-frame entry
- * (`launch_cudaProcess`, include/cuda/render.cuh:117-130, defined in
+ * (`launch_cudaProcess`, include/cuda/render.cuh:41-57, defined in
  * src/cuda/render/CUDA2OpenGL.cu:68-74) and its frame loader
  * (`io::LoadFrame`, src/cuda/common/slice_io.cu:143-275), plus native
  * status-returning calls for the batched persistent decode and the
@@ -55,7 +55,7 @@ typedef struct Points {
 
 /* Opaque device-resident per-patch decoder state: replaces the array of
  * `vvc::client::octree::InvertRAHTOctree` objects
- * (include/cuda/octree.cuh:413-454) that the reference caller allocates
+ * (include/cuda/octree.cuh:130-171) that the reference caller allocates
  * with cudaMalloc and fills with default-constructed host copies
  * (src/cuda/common/manager.cc:235-238, test/test_decode_only.cc:51-55).
  * Holds, per patch index, the intra geometry bytes, the reference colours
@@ -70,7 +70,7 @@ void pvvc_state_free(pvvc_state* state);
  * synchronises the library stream). */
 int pvvc_state_patch_status(const pvvc_state* state, int patch);
 
-/* Drop-in for launch_cudaProcess (include/cuda/render.cuh:117-130,
+/* Drop-in for launch_cudaProcess (include/cuda/render.cuh:41-57,
  * src/cuda/render/CUDA2OpenGL.cu:68-74).  Same argument meaning: every
  * pointer is caller-owned device memory; geometry[s]/color[s] hold the
  * already zstd-unwrapped bytes of slice s; mv[s] is 16 row-major floats;

@@ -5,7 +5,7 @@ include/pvvc_hip.h; this module wraps that ABI with ctypes and numpy:
 
 * Frame             -- a loaded .frame (io::LoadFrame, src/cuda/common/slice_io.cu:143-275)
 * State, decode_dropin -- the reference's launch_cudaProcess call surface
-                        (include/cuda/render.cuh:117-130) with its device arrays
+                        (include/cuda/render.cuh:41-57) with its device arrays
 * Decoder, Batch    -- native batched decode (per-frame, persistent, graph)
 * synth_stream      -- seeded synthetic humanoid streams (stream writer)
 """
