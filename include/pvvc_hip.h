@@ -198,6 +198,15 @@ int pvvc_batch_status(pvvc_batch* batch, int* status, int n);
 /* Resets every patch state slot (drops GoP state). */
 int pvvc_decoder_reset(pvvc_decoder* dec);
 
+/* Autotuner: picks waves per slice (1/2/4), the LDS budget per slice
+ * (4-64 KiB) and, for the persistent mode, workgroups per CU and the queue
+ * chunk by coordinate descent, timing `reps` decodes of `batch` per
+ * candidate (HIP events; median).  Sets the decoder's tuning to the best
+ * and returns it with its time; the outputs hold that configuration's
+ * decode.  Synchronises the library stream. */
+int pvvc_decoder_autotune(pvvc_decoder* dec, pvvc_batch* batch, int mode, int output, float* xyz, uint8_t* rgb,
+                          Points* points, int reps, pvvc_tuning* best, double* best_ms);
+
 /* ------------------------------------------------------------------------ */
 /* Device memory / timing helpers (plumbing for callers without a HIP        */
 /* binding of their own).                                                    */

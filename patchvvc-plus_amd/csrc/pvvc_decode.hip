@@ -15,6 +15,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 namespace pvvc {
@@ -1177,6 +1178,181 @@ bool state_reset(StateInfo& si) {
 
 }  // namespace
 
+namespace pvvc {
+
+/* Queue segment: one persistent launch. */
+struct Segment {
+    uint32_t begin, count;       /* range in order[] */
+    uint32_t dbegin, dcount;     /* deferred predictive items, range in defer[] */
+    uint64_t ws_need;            /* per-item workspace bound */
+};
+
+/* Host-side plan of a batch image: [items][order][status][defer][frame blobs],
+ * 256-B aligned sections.  Frame f's slices use state slots
+ * slot_base[f] + index[s]. */
+struct BatchPlan {
+    std::vector<ItemDev> items;
+    std::vector<uint64_t> geo_at, col_at;  /* blob offsets in the image (UINT64_MAX: none) */
+    std::vector<uint32_t> order, defer;
+    std::vector<Segment> segs;
+    std::vector<int64_t> frame_off, frame_pts;
+    std::vector<uint32_t> frame_first, frame_count;
+    std::vector<size_t> frame_data_at;
+    uint32_t n_items = 0;
+    int64_t total_points = 0, input_bytes = 0, state_bytes = 0;
+    size_t o_items = 0, o_order = 0, o_status = 0, o_defer = 0, o_data = 0, total = 0;
+};
+
+/* Workspace bound of one item (coefficients, first-child and weight arrays,
+ * top-down buffers); geo_bytes is the geometry blob it expands. */
+inline uint64_t item_ws_bound(uint32_t n, uint64_t geo_bytes) {
+    return 12ull * n + 2 * (4 * (geo_bytes + 1) + 16) + 48ull * n + 512;
+}
+
+/* Plans the batch of frames [0, n_frames) in decode order.
+ *  - a predictive item whose patch's intra item is earlier in the same
+ *    segment expands that item's geometry and is deferred (its reference is
+ *    added by the finish pass); otherwise it reads the state slot;
+ *  - a patch's new intra slice after earlier items of that patch starts a new
+ *    segment, so no launch rewrites state an earlier item still reads;
+ *  - queue order per segment: point count descending, an intra item keyed by
+ *    the largest size of its patch, intra first on ties.
+ * intra_geo (per slot, may be null) tracks the latest intra geometry size for
+ * the state-read accounting. */
+int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_base, const int* slot_count,
+               const uint32_t* geo_cap, uint32_t* intra_geo, BatchPlan& p) {
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    uint32_t n_items = 0;
+    size_t data_bytes = 0;
+    for (int f = 0; f < n_frames; ++f) {
+        if (!frames[f]) return PVVC_ERR_INVALID_ARG;
+        n_items += frames[f]->slice_cnt;
+        data_bytes += al(frames[f]->data.size());
+    }
+    p = BatchPlan{};
+    p.n_items = n_items;
+    p.o_items = 0;
+    p.o_order = al(sizeof(ItemDev) * n_items);
+    p.o_status = al(p.o_order + 4ull * n_items);
+    p.o_defer = al(p.o_status + 4ull * n_items);
+    p.o_data = al(p.o_defer + 4ull * n_items);
+    p.total = p.o_data + data_bytes;
+    p.items.assign(n_items, ItemDev{});
+    p.geo_at.assign(n_items, UINT64_MAX);
+    p.col_at.assign(n_items, UINT64_MAX);
+    std::vector<uint32_t> gcap_of(n_items, 0);
+    std::map<int, uint32_t> intra_item;      /* slot -> intra item in this segment */
+    std::map<int, int> seen;                 /* slots with items in this segment */
+    std::vector<uint32_t> seg_starts{0};
+    size_t data_at = p.o_data;
+    int64_t pts_acc = 0;
+    uint32_t it = 0;
+    for (int f = 0; f < n_frames; ++f) {
+        const pvvc_frame* fr = frames[f];
+        p.frame_first.push_back(it);
+        p.frame_count.push_back(fr->slice_cnt);
+        p.frame_off.push_back(pts_acc);
+        p.frame_pts.push_back(static_cast<int64_t>(fr->points));
+        p.frame_data_at.push_back(data_at);
+        bool new_seg = false;
+        for (uint32_t s = 0; s < fr->slice_cnt; ++s)
+            if (!(fr->type[s] & PVVC_TYPE_PREDICT) && seen.count(slot_base[f] + fr->index[s])) new_seg = true;
+        if (new_seg && it != 0) {
+            seg_starts.push_back(it);
+            intra_item.clear();
+            seen.clear();
+        }
+        int64_t inner = 0;
+        for (uint32_t s = 0; s < fr->slice_cnt; ++s, ++it) {
+            ItemDev& x = p.items[it];
+            std::memset(&x, 0, sizeof(x));
+            const int pi = fr->index[s];
+            if (pi < 0 || pi >= slot_count[f]) return PVVC_ERR_INVALID_ARG;
+            const int slot = slot_base[f] + pi;
+            if (fr->geo_size[s]) p.geo_at[it] = data_at + fr->geo_off[s];
+            if (fr->col_size[s]) p.col_at[it] = data_at + fr->col_off[s];
+            x.geo_size = fr->geo_size[s];
+            x.col_size = fr->col_size[s];
+            x.n = fr->size[s];
+            x.patch = slot;
+            x.type = fr->type[s];
+            x.qp = fr->qp[s];
+            x.out_off = static_cast<uint64_t>(pts_acc + inner);
+            for (int k = 0; k < 12; ++k) x.mv[k] = fr->mv[16 * s + k];
+            gcap_of[it] = geo_cap[f];
+            if (x.type & PVVC_TYPE_PREDICT) {
+                if (intra_geo) p.state_bytes += static_cast<int64_t>(intra_geo[slot]) + 12ll * x.n;
+                auto ii = intra_item.find(slot);
+                if (ii != intra_item.end()) {
+                    p.geo_at[it] = p.geo_at[ii->second];
+                    x.geo_size = p.items[ii->second].geo_size;
+                    x.wait = 1;
+                } else {
+                    p.geo_at[it] = UINT64_MAX;
+                    x.geo_size = 0;
+                }
+            } else {
+                if (intra_geo) intra_geo[slot] = x.geo_size;
+                intra_item[slot] = it;
+            }
+            seen[slot] = f;
+            inner += fr->size[s];
+        }
+        pts_acc += inner;
+        data_at += al(fr->data.size());
+        p.input_bytes += 82ll * fr->slice_cnt;
+        for (uint32_t s = 0; s < fr->slice_cnt; ++s) p.input_bytes += fr->geo_size[s] + fr->col_size[s];
+    }
+    p.total_points = pts_acc;
+    p.order.assign(n_items, 0);
+    seg_starts.push_back(n_items);
+    for (size_t si = 0; si + 1 < seg_starts.size(); ++si) {
+        const uint32_t lo = seg_starts[si], hi = seg_starts[si + 1];
+        std::vector<uint32_t> ids(hi - lo);
+        std::iota(ids.begin(), ids.end(), lo);
+        std::map<int, uint32_t> patch_max;
+        for (uint32_t i : ids) patch_max[p.items[i].patch] = std::max(patch_max[p.items[i].patch], p.items[i].n);
+        std::stable_sort(ids.begin(), ids.end(), [&](uint32_t a, uint32_t c) {
+            const ItemDev &A = p.items[a], &C = p.items[c];
+            const bool pa = A.type & PVVC_TYPE_PREDICT, pc = C.type & PVVC_TYPE_PREDICT;
+            const uint32_t ka = pa ? A.n : patch_max[A.patch];
+            const uint32_t kc = pc ? C.n : patch_max[C.patch];
+            if (ka != kc) return ka > kc;
+            if (pa != pc) return !pa;
+            return a < c;
+        });
+        uint64_t need = 0;
+        for (uint32_t i : ids) {
+            const ItemDev& x = p.items[i];
+            need = std::max(need, item_ws_bound(x.n, x.geo_size ? x.geo_size : gcap_of[i]));
+        }
+        std::copy(ids.begin(), ids.end(), p.order.begin() + lo);
+        const uint32_t dlo = static_cast<uint32_t>(p.defer.size());
+        for (uint32_t i = lo; i < hi; ++i)
+            if (p.items[i].wait) p.defer.push_back(i);
+        p.segs.push_back(Segment{lo, hi - lo, dlo, static_cast<uint32_t>(p.defer.size()) - dlo,
+                                 (need + 255) & ~uint64_t(255)});
+    }
+    return PVVC_OK;
+}
+
+/* Writes the planned image for a device copy at dev_base into host (p.total bytes). */
+void fill_batch_image(const BatchPlan& p, const pvvc_frame* const* frames, uint8_t* dev_base, uint8_t* host) {
+    std::vector<ItemDev> items = p.items;
+    for (uint32_t i = 0; i < p.n_items; ++i) {
+        items[i].geo = p.geo_at[i] == UINT64_MAX ? nullptr : dev_base + p.geo_at[i];
+        items[i].col = p.col_at[i] == UINT64_MAX ? nullptr : dev_base + p.col_at[i];
+    }
+    std::memcpy(host + p.o_items, items.data(), sizeof(ItemDev) * p.n_items);
+    std::memcpy(host + p.o_order, p.order.data(), 4ull * p.n_items);
+    std::memset(host + p.o_status, 0, 4ull * p.n_items);
+    if (!p.defer.empty()) std::memcpy(host + p.o_defer, p.defer.data(), 4ull * p.defer.size());
+    for (size_t f = 0; f < p.frame_data_at.size(); ++f)
+        std::memcpy(host + p.frame_data_at[f], frames[f]->data.data(), frames[f]->data.size());
+}
+
+}  // namespace pvvc
+
 /* ---- native decoder / batch ---- */
 struct pvvc_decoder {
     StateInfo st;
@@ -1191,13 +1367,8 @@ struct pvvc_decoder {
                                         seen by pvvc_batch_create (state-read accounting) */
 };
 
-struct Segment {
-    uint32_t begin, count;       /* range in order[] */
-    uint32_t dbegin, dcount;     /* deferred predictive items, range in defer[] */
-    uint64_t ws_need;            /* per-item workspace bound */
-};
-
 struct pvvc_batch {
+    BatchPlan plan;
     pvvc_decoder* dec = nullptr;
     int n_frames = 0;
     std::vector<int64_t> frame_off, frame_pts;
@@ -1554,157 +1725,41 @@ int pvvc_decoder_reset(pvvc_decoder* d) {
 
 int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_frames, pvvc_batch** out) {
     if (!d || !frames || n_frames <= 0 || !out) return PVVC_ERR_INVALID_ARG;
+    std::vector<int> base(n_frames, 0), count(n_frames, d->st.n);
+    std::vector<uint32_t> gcap(n_frames, d->st.geo_cap);
     auto* b = new pvvc_batch();
-    b->dec = d;
-    b->n_frames = n_frames;
-    /* host image: [items][order][status][frame blobs] */
-    uint32_t n_items = 0;
-    size_t data_bytes = 0;
-    for (int f = 0; f < n_frames; ++f) {
-        if (!frames[f]) {
-            delete b;
-            return PVVC_ERR_INVALID_ARG;
-        }
-        n_items += frames[f]->slice_cnt;
-        data_bytes += (frames[f]->data.size() + 255) & ~size_t(255);
+    BatchPlan& pl = b->plan;
+    int rc = plan_batch(frames, n_frames, base.data(), count.data(), gcap.data(), d->intra_geo.data(), pl);
+    if (rc != PVVC_OK) {
+        delete b;
+        return rc;
     }
-    b->n_items = n_items;
-    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t o_items = 0, o_order = al(sizeof(ItemDev) * n_items), o_status = al(o_order + 4ull * n_items);
-    const size_t o_defer = al(o_status + 4ull * n_items);
-    const size_t o_data = al(o_defer + 4ull * n_items);
-    const size_t total = o_data + data_bytes;
-    if (!hip_ok(hipMalloc(&b->dev, std::max<size_t>(total, 256)))) {
+    if (!hip_ok(hipMalloc(&b->dev, std::max<size_t>(pl.total, 256)))) {
         delete b;
         return PVVC_ERR_HIP;
     }
-    std::vector<uint8_t> h(total, 0);
-    std::vector<ItemDev> items(n_items);
-    std::vector<uint32_t> patch_of(n_items);
-    /* per patch: the item index of the intra slice seen in this batch, if any */
-    std::map<int, uint32_t> intra_item;
-    std::map<int, int> last_frame_of_patch;
-    size_t data_at = 0;
-    int64_t pts_acc = 0;
-    uint32_t it = 0;
-    bool split_needed = false;
-    std::vector<uint32_t> seg_starts{0};
-    for (int f = 0; f < n_frames; ++f) {
-        const pvvc_frame* fr = frames[f];
-        b->frame_first.push_back(it);
-        b->frame_count.push_back(fr->slice_cnt);
-        b->frame_off.push_back(pts_acc);
-        b->frame_pts.push_back(static_cast<int64_t>(fr->points));
-        uint8_t* dbase = b->dev + o_data + data_at;
-        std::memcpy(&h[o_data + data_at], fr->data.data(), fr->data.size());
-        /* A patch's new intra slice after earlier items of that patch in the
-         * same batch starts a new segment (launch), so that it never rewrites
-         * state that an earlier predictive item still reads. */
-        bool new_seg = false;
-        for (uint32_t s = 0; s < fr->slice_cnt; ++s) {
-            int p = fr->index[s];
-            if (!(fr->type[s] & PVVC_TYPE_PREDICT) && last_frame_of_patch.count(p)) new_seg = true;
-        }
-        if (new_seg && it != 0) {
-            seg_starts.push_back(it);
-            intra_item.clear();
-            last_frame_of_patch.clear();
-            split_needed = true;
-        }
-        int64_t inner = 0;
-        for (uint32_t s = 0; s < fr->slice_cnt; ++s, ++it) {
-            ItemDev& x = items[it];
-            std::memset(&x, 0, sizeof(x));
-            const int p = fr->index[s];
-            x.geo = fr->geo_size[s] ? dbase + fr->geo_off[s] : nullptr;
-            x.geo_size = fr->geo_size[s];
-            x.col = fr->col_size[s] ? dbase + fr->col_off[s] : nullptr;
-            x.col_size = fr->col_size[s];
-            x.n = fr->size[s];
-            x.patch = p;
-            x.type = fr->type[s];
-            x.qp = fr->qp[s];
-            x.out_off = static_cast<uint64_t>(pts_acc + inner);
-            for (int k = 0; k < 12; ++k) x.mv[k] = fr->mv[16 * s + k];
-            if (p < 0 || p >= d->st.n) {
-                pvvc_dev_free(b->dev);
-                delete b;
-                return PVVC_ERR_INVALID_ARG;
-            }
-            if (x.type & PVVC_TYPE_PREDICT)
-                b->state_bytes += static_cast<int64_t>(d->intra_geo[p]) + 12ll * x.n;
-            else
-                d->intra_geo[p] = x.geo_size;
-            if (x.type & PVVC_TYPE_PREDICT) {
-                auto ii = intra_item.find(p);
-                if (ii != intra_item.end()) {
-                    const ItemDev& in = items[ii->second];
-                    x.geo = in.geo;
-                    x.geo_size = in.geo_size;
-                    x.wait = 1;
-                } else {
-                    x.geo = nullptr;
-                    x.geo_size = 0;
-                }
-            } else {
-                intra_item[p] = it;
-            }
-            last_frame_of_patch[p] = f;
-            patch_of[it] = p;
-            inner += fr->size[s];
-        }
-        pts_acc += inner;
-        data_at += (fr->data.size() + 255) & ~size_t(255);
-        b->input_bytes += 82ll * fr->slice_cnt;
-        for (uint32_t s = 0; s < fr->slice_cnt; ++s) b->input_bytes += fr->geo_size[s] + fr->col_size[s];
-    }
-    (void)split_needed;
-    b->total_points = pts_acc;
-    /* queue order per segment: longest first; intra before predictive of equal length */
-    std::vector<uint32_t> order(n_items), defer;
-    seg_starts.push_back(n_items);
-    for (size_t sidx = 0; sidx + 1 < seg_starts.size(); ++sidx) {
-        uint32_t lo = seg_starts[sidx], hi = seg_starts[sidx + 1];
-        std::vector<uint32_t> ids(hi - lo);
-        std::iota(ids.begin(), ids.end(), lo);
-        /* key: a patch's intra item inherits the largest size of its patch */
-        std::map<int, uint32_t> patch_max;
-        for (uint32_t i : ids) patch_max[patch_of[i]] = std::max(patch_max[patch_of[i]], items[i].n);
-        std::stable_sort(ids.begin(), ids.end(), [&](uint32_t a, uint32_t c) {
-            const ItemDev &A = items[a], &C = items[c];
-            uint32_t ka = (A.type & PVVC_TYPE_PREDICT) ? A.n : patch_max[patch_of[a]];
-            uint32_t kc = (C.type & PVVC_TYPE_PREDICT) ? C.n : patch_max[patch_of[c]];
-            if (ka != kc) return ka > kc;
-            bool pa = A.type & PVVC_TYPE_PREDICT, pc = C.type & PVVC_TYPE_PREDICT;
-            if (pa != pc) return !pa;
-            return a < c;
-        });
-        uint64_t need = 0;
-        for (uint32_t i : ids) {
-            const ItemDev& x = items[i];
-            uint64_t g = x.geo_size ? x.geo_size : d->st.geo_cap;
-            uint64_t bound = 12ull * x.n + 2 * (4 * (g + 1) + 16) + 48ull * x.n + 512;
-            need = std::max(need, bound);
-        }
-        std::copy(ids.begin(), ids.end(), order.begin() + lo);
-        const uint32_t dlo = static_cast<uint32_t>(defer.size());
-        for (uint32_t i = lo; i < hi; ++i)
-            if (items[i].wait) defer.push_back(i);
-        b->segs.push_back(Segment{lo, hi - lo, dlo, static_cast<uint32_t>(defer.size()) - dlo,
-                                  (need + 255) & ~uint64_t(255)});
-    }
-    std::memcpy(&h[o_items], items.data(), sizeof(ItemDev) * n_items);
-    std::memcpy(&h[o_order], order.data(), 4ull * n_items);
-    if (!defer.empty()) std::memcpy(&h[o_defer], defer.data(), 4ull * defer.size());
-    if (!hip_ok(hipMemcpy(b->dev, h.data(), total, hipMemcpyHostToDevice))) {
+    std::vector<uint8_t> h(pl.total, 0);
+    fill_batch_image(pl, frames, b->dev, h.data());
+    if (!hip_ok(hipMemcpy(b->dev, h.data(), pl.total, hipMemcpyHostToDevice))) {
         hipFree(b->dev);
         delete b;
         return PVVC_ERR_HIP;
     }
-    b->items = reinterpret_cast<ItemDev*>(b->dev + o_items);
-    b->order = reinterpret_cast<uint32_t*>(b->dev + o_order);
-    b->defer = reinterpret_cast<uint32_t*>(b->dev + o_defer);
-    b->status = reinterpret_cast<int*>(b->dev + o_status);
+    b->dec = d;
+    b->n_frames = n_frames;
+    b->n_items = pl.n_items;
+    b->total_points = pl.total_points;
+    b->input_bytes = pl.input_bytes;
+    b->state_bytes = pl.state_bytes;
+    b->frame_off = pl.frame_off;
+    b->frame_pts = pl.frame_pts;
+    b->frame_first = pl.frame_first;
+    b->frame_count = pl.frame_count;
+    b->segs = pl.segs;
+    b->items = reinterpret_cast<ItemDev*>(b->dev + pl.o_items);
+    b->order = reinterpret_cast<uint32_t*>(b->dev + pl.o_order);
+    b->defer = reinterpret_cast<uint32_t*>(b->dev + pl.o_defer);
+    b->status = reinterpret_cast<int*>(b->dev + pl.o_status);
     *out = b;
     return PVVC_OK;
 }
@@ -1837,6 +1892,84 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
         hipEventRecord(ev.second, g_stream);
     }
     return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
+}
+
+/* Autotuner: coordinate descent over the launch knobs on a real batch.
+ * Each candidate decodes the batch once untimed, then `reps` times; its
+ * cost is the median of the summed event times of all launches of a call.
+ * Decoding a batch again is idempotent (intra slices rewrite the state they
+ * read), so the output equals a single decode's. */
+int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, float* xyz, uint8_t* rgb,
+                          Points* points, int reps, pvvc_tuning* best, double* best_ms) {
+    if (!d || !b || b->dec != d) return PVVC_ERR_INVALID_ARG;
+    if (mode != PVVC_MODE_PER_FRAME && mode != PVVC_MODE_PERSISTENT && mode != PVVC_MODE_GRAPH)
+        return PVVC_ERR_INVALID_ARG;
+    reps = std::max(1, std::min(reps, 50));
+    static const int kWaves[] = {1, 2, 4};
+    static const int kLds[] = {4096, 8192, 16384, 24576, 32768, 49152, 65536};
+    static const int kWgs[] = {2, 4, 6, 8, 12, 16};
+    static const int kChunk[] = {1, 2, 4, 8};
+    const bool persistent = mode == PVVC_MODE_PERSISTENT;
+    std::map<std::tuple<int, int, int, int>, double> seen;
+    auto measure = [&](const pvvc_tuning& t, double& ms) -> int {
+        auto key = std::make_tuple(t.waves_per_patch, t.lds_bytes, t.wgs_per_cu, t.queue_chunk);
+        auto f = seen.find(key);
+        if (f != seen.end()) {
+            ms = f->second;
+            return PVVC_OK;
+        }
+        d->tuning = t;
+        int rc = pvvc_decode_batch(d, b, mode, output, xyz, rgb, points);
+        if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(g_stream))) rc = PVVC_ERR_HIP;
+        std::vector<double> v;
+        for (int r = 0; r < reps && rc == PVVC_OK; ++r) {
+            rc = pvvc_decode_batch(d, b, mode, output, xyz, rgb, points);
+            if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(g_stream))) rc = PVVC_ERR_HIP;
+            v.push_back(pvvc_last_kernel_ms());
+        }
+        if (rc != PVVC_OK) return rc;
+        std::sort(v.begin(), v.end());
+        ms = v[v.size() / 2];
+        seen[key] = ms;
+        return PVVC_OK;
+    };
+    pvvc_tuning cur = d->tuning;
+    double cur_ms = 0;
+    int rc = measure(cur, cur_ms);
+    if (rc != PVVC_OK) return rc;
+    for (int pass = 0; pass < 3; ++pass) {
+        const pvvc_tuning start = cur;
+        for (int dim = 0; dim < 4; ++dim) {
+            if (!persistent && dim >= 2) break; /* grid shape is the slice count there */
+            const int* vals = dim == 0 ? kWaves : dim == 1 ? kLds : dim == 2 ? kWgs : kChunk;
+            const int nv = dim == 0 ? 3 : dim == 1 ? 7 : dim == 2 ? 6 : 4;
+            for (int k = 0; k < nv; ++k) {
+                pvvc_tuning t = cur;
+                int* field = dim == 0 ? &t.waves_per_patch : dim == 1 ? &t.lds_bytes : dim == 2 ? &t.wgs_per_cu
+                                                                                        : &t.queue_chunk;
+                if (*field == vals[k]) continue;
+                *field = vals[k];
+                double ms = 0;
+                rc = measure(t, ms);
+                if (rc != PVVC_OK) {
+                    d->tuning = cur;
+                    return rc;
+                }
+                if (ms < cur_ms * 0.99) {
+                    cur = t;
+                    cur_ms = ms;
+                }
+            }
+        }
+        if (std::memcmp(&start, &cur, sizeof(cur)) == 0) break;
+    }
+    d->tuning = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    /* leave the outputs as the chosen configuration writes them */
+    rc = pvvc_decode_batch(d, b, mode, output, xyz, rgb, points);
+    if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(g_stream))) rc = PVVC_ERR_HIP;
+    return rc;
 }
 
 }  // extern "C"

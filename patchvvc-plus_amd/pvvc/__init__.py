@@ -353,6 +353,21 @@ class Decoder:
         _check(hip_lib().pvvc_decode_batch(self.handle, batch.handle, mode, out_mode, p(out.xyz), p(out.rgb),
                                            p(out.pts)), "pvvc_decode_batch")
 
+    def autotune(self, batch: Batch, mode: int = MODE_PERSISTENT, out: Outputs | None = None,
+                 reps: int = 3) -> tuple[dict, float]:
+        """Runs the native autotuner on `batch`; returns (tuning, ms per decode call)
+        and leaves the decoder set to that tuning."""
+        own = out is None
+        if own:
+            out = self.outputs(batch, OUT_NATIVE)
+        out_mode = OUT_NATIVE if out.xyz is not None else OUT_POINTS
+        p = lambda b: C.c_void_p(b.ptr) if b is not None else None  # noqa: E731
+        t = Tuning()
+        ms = C.c_double()
+        _check(hip_lib().pvvc_decoder_autotune(self.handle, batch.handle, mode, out_mode, p(out.xyz), p(out.rgb),
+                                               p(out.pts), reps, C.byref(t), C.byref(ms)), "pvvc_decoder_autotune")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}, float(ms.value)
+
     def decode_host(self, batch: Batch, mode: int = MODE_PERSISTENT, output: int = OUT_NATIVE):
         """Decodes and reads back: (xyz f32[N,3], rgb u8[N,3]) or Points[N]."""
         out = self.outputs(batch, output)

@@ -87,6 +87,8 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_decoder_set_tuning", C.c_int, c_vp, C.POINTER(Tuning))
     _sig(L, "pvvc_decoder_get_tuning", C.c_int, c_vp, C.POINTER(Tuning))
     _sig(L, "pvvc_decoder_reset", C.c_int, c_vp)
+    _sig(L, "pvvc_decoder_autotune", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp, C.c_int,
+         C.POINTER(Tuning), C.POINTER(C.c_double))
     _sig(L, "pvvc_batch_create", C.c_int, c_vp, C.POINTER(c_vp), C.c_int, C.POINTER(c_vp))
     _sig(L, "pvvc_batch_free", None, c_vp)
     _sig(L, "pvvc_batch_frames", C.c_int, c_vp)

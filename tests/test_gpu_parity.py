@@ -84,3 +84,29 @@ def test_tuning_variants_identical(pvvc, c0_files, c0_oracle, waves, lds):
         o, n = b.frame_offset[i], b.frame_points[i]
         np.testing.assert_array_equal(xyz[o:o + n], ref["xyz"])
         np.testing.assert_array_equal(rgb[o:o + n], ref["rgb"])
+
+
+def test_autotune_keeps_output(pvvc, c0_files, c0_oracle):
+    """The autotuner returns a configuration from its search space, sets it,
+    and the decode under it still equals the oracle."""
+    _say("autotune c0")
+    frames = [pvvc.Frame.load(str(p)) for p in c0_files]
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames)
+    dec = pvvc.Decoder(n_patches, max_pts)
+    b = dec.batch(frames)
+    out = dec.outputs(b, pvvc.OUT_NATIVE)
+    best, ms = dec.autotune(b, pvvc.MODE_PERSISTENT, out, reps=2)
+    _say("  best", best, "ms", ms)
+    assert best == dec.tuning()
+    assert best["waves_per_patch"] in (1, 2, 4)
+    assert 4096 <= best["lds_bytes"] <= 65536
+    assert ms > 0
+    n = b.total_points
+    xyz = out.xyz.read(np.float32, 3 * n).reshape(n, 3)
+    rgb = out.rgb.read(np.uint8, 3 * n).reshape(n, 3)
+    assert (b.status() == 0).all()
+    for i, ref in enumerate(c0_oracle):
+        o, k = b.frame_offset[i], b.frame_points[i]
+        np.testing.assert_array_equal(xyz[o:o + k], ref["xyz"])
+        np.testing.assert_array_equal(rgb[o:o + k], ref["rgb"])
