@@ -106,6 +106,37 @@ def measure_traffic(args) -> dict | None:
                            f"WRITE_SIZE {res['copy_write']:.1f} KiB"}
 
 
+def serving_leg(args, pvvc) -> dict:
+    """Serving workload: streams of mixed density (one third each 100k pts /
+    9-bit, 400k / 10-bit, 1M / 10-bit; GoP 8, qp 4), Poisson arrivals at 30 fps
+    per stream, continuous batching on the server, GoP state resident."""
+    from pvvc import serve
+    n = args.serve_streams
+    kinds = ["c3_100k", "c3_400k", "c3_1m"]
+    counts = [n // 3, n // 3, n - 2 * (n // 3)]
+    t0 = time.time()
+    gops = {}
+    for i, kind in enumerate(kinds):
+        imgs = pvvc.synth_stream(kind, n_frames=8, seed=100 + i, threads=min(16, os.cpu_count() or 8))
+        gops[kind] = [pvvc.Frame.parse(b) for b in imgs]
+    log(f"serving: generated {len(kinds)} GoPs in {time.time() - t0:.1f}s")
+    specs = []
+    for kind, c in zip(kinds, counts):
+        specs += [serve.StreamSpec(kind, gops[kind]) for _ in range(c)]
+    srv = pvvc.Server(lanes=args.serve_lanes)
+    st = serve.run(srv, specs, fps=30.0, duration=args.serve_seconds, seed=args.seed)
+    srv.close()
+    st["pts_per_frame"] = {k: int(np.mean([f.points for f in g])) for k, g in gops.items()}
+    st["streams_per_kind"] = dict(zip(kinds, counts))
+    st["sustained"] = bool(st["frames_completed"] == st["frames_offered"] and st["frames_failed"] == 0
+                           and st["p99_ms"] <= 33.0)
+    for k in ("p50_ms", "p99_ms", "max_ms", "mpts_per_s", "offered_mpts_per_s", "submit_late_p99_ms",
+              "queue_wait_p50_ms", "queue_wait_p99_ms", "service_p50_ms", "service_p99_ms", "frames_per_batch"):
+        st[k] = round(float(st[k]), 3)
+    log("serving", st)
+    return st
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -131,7 +162,11 @@ def main() -> int:
     ap.add_argument("--wgs-per-cu", type=int, default=0)
     ap.add_argument("--queue-chunk", type=int, default=0)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
-    ap.add_argument("--only", choices=["all", "batched", "perframe"], default="all",
+    ap.add_argument("--no-serving", action="store_true", help="skip the multi-stream serving leg")
+    ap.add_argument("--serve-streams", type=int, default=64)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--only", choices=["all", "batched", "perframe", "serving"], default="all",
                     help="restrict the GPU legs (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
@@ -161,6 +196,9 @@ def main() -> int:
 
     info = pvvc.device_info()
     log("device", info)
+    if args.only == "serving":
+        print(json.dumps({"serving": serving_leg(args, pvvc)}))
+        return 0
     gop = pvvc.SYNTH_PRESETS[args.preset]["gop"]
     t0 = time.time()
     images = pvvc.synth_stream(args.preset, n_frames=gop * args.gops, seed=args.seed, threads=min(16, os.cpu_count() or 8))
@@ -246,6 +284,11 @@ def main() -> int:
     frame_kernel_ms = frame_kernel_ms or [float("nan")]
     log(f"per-frame drop-in: {np.mean(frame_kernel_ms):.3f} ms kernel, p50 {pct(lat, 50):.3f} p99 {pct(lat, 99):.3f} ms,"
         f" {pf_fps:.1f} fps")
+
+    # ---------------- serving loop: many concurrent streams ----------------
+    serving = None
+    if not args.no_serving and args.only == "all":
+        serving = serving_leg(args, pvvc)
 
     # ---------------- CPU baseline: oracle on a bounded sample ----------------
     cpu, accuracy = None, None
@@ -340,6 +383,7 @@ def main() -> int:
         "state_bytes_per_step": int(np.mean(state_bytes)),
         "rlgr_symbols_per_s_per_cu": round(sym_per_s_cu, 1),
         "accuracy": accuracy,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -208,6 +208,73 @@ int pvvc_decoder_autotune(pvvc_decoder* dec, pvvc_batch* batch, int mode, int ou
                           Points* points, int reps, pvvc_tuning* best, double* best_ms);
 
 /* ------------------------------------------------------------------------ */
+/* Serving loop: many concurrent streams, continuous batching.  The          */
+/* reference client decodes one stream with a loader thread and a decoder   */
+/* thread around launch_cudaProcess (src/cuda/common/manager.cc:145-310);    */
+/* the server generalises that decoder thread to many streams.              */
+/* ------------------------------------------------------------------------ */
+typedef struct pvvc_server pvvc_server;
+
+typedef struct pvvc_server_config {
+    uint64_t hbm_budget;        /* bytes for admitted streams' state + output rings
+                                   (0: 80% of the free HBM after the fixed buffers) */
+    int lanes;                  /* batches in flight at once, each on its own HIP stream (default 3) */
+    int max_slots;              /* patch state slots over all streams (default 65536) */
+    int max_points_per_patch;   /* largest patch any stream may send (default 4096) */
+    int64_t out_pool_points;    /* output pool shared by the streams' rings (default 64 Mi points) */
+    int64_t max_batch_points;   /* points per batch (default 16 Mi) */
+    pvvc_tuning tuning;         /* launch shape of the persistent decode */
+} pvvc_server_config;
+
+typedef struct pvvc_completion {
+    int stream_id;
+    int status;                 /* first non-OK slice status of the frame, or OK */
+    int64_t seq;                /* frame number within its stream (submission order) */
+    uint64_t tag;               /* caller's tag from pvvc_server_submit */
+    int64_t points;
+    const float* xyz;           /* device: points x 3 floats, valid until pvvc_server_release */
+    const uint8_t* rgb;         /* device: points x 3 bytes */
+    int out_slot;               /* ring slot to hand back with pvvc_server_release */
+    double arrival_s, launch_s, done_s;  /* pvvc_server_clock() times */
+} pvvc_completion;
+
+typedef struct pvvc_server_stats {
+    int64_t frames_done, points_done, batches;
+    uint64_t hbm_budget, hbm_used;
+    int streams_active, streams_waiting;
+    double p50_ms, p99_ms, max_ms;  /* arrival -> completion, per frame */
+    double mpts_per_s;              /* points done / (last completion - first arrival) */
+} pvvc_server_stats;
+
+/* Monotonic clock (seconds) for arrival stamps. */
+double pvvc_server_clock(void);
+int pvvc_server_create(const pvvc_server_config* cfg, pvvc_server** out);
+/* Stops the loop after the batches in flight; frees everything. */
+int pvvc_server_destroy(pvvc_server* srv);
+/* Opens a stream.  It is admitted at once if its footprint (state slots for
+ * max_patches patches of up to max_points_per_patch points, plus out_ring
+ * output frames of max_points_per_frame points) fits the budget and no
+ * stream waits before it; else it waits (state 0) and is admitted, first in
+ * first out, when closed streams free memory.  Frames may be submitted
+ * while it waits.  PVVC_ERR_CAPACITY if it could never fit. */
+int pvvc_server_open_stream(pvvc_server* srv, int max_patches, int max_points_per_patch, int64_t max_points_per_frame,
+                            int out_ring, int* stream_id);
+/* 0 waiting for admission, 1 active, 2 closing, 3 closed; -1 unknown id. */
+int pvvc_server_stream_state(pvvc_server* srv, int stream_id);
+/* Queues a frame (decode order).  The frame must stay alive until its
+ * completion has been polled.  arrival_s stamps the latency start. */
+int pvvc_server_submit(pvvc_server* srv, int stream_id, const pvvc_frame* frame, double arrival_s, uint64_t tag);
+/* Closes a stream: its queued frames still decode, then its memory is freed
+ * and waiting streams are admitted. */
+int pvvc_server_close_stream(pvvc_server* srv, int stream_id);
+/* Takes up to `max` completions, waiting up to timeout_ms (-1: forever). */
+int pvvc_server_poll(pvvc_server* srv, pvvc_completion* out, int max, int timeout_ms);
+/* Hands an output ring slot back; the stream's next frame may then use it. */
+int pvvc_server_release(pvvc_server* srv, int stream_id, int out_slot);
+int pvvc_server_get_stats(pvvc_server* srv, pvvc_server_stats* out);
+int pvvc_server_reset_stats(pvvc_server* srv);
+
+/* ------------------------------------------------------------------------ */
 /* Device memory / timing helpers (plumbing for callers without a HIP        */
 /* binding of their own).                                                    */
 /* ------------------------------------------------------------------------ */

@@ -65,13 +65,16 @@ def build(jobs: int = 4) -> list[Path]:
         obj / "frame_io.o": [CXX, *CXX_FLAGS, f"-I{inc}", "-c", str(CSRC / "frame_io.cpp"), "-o", str(obj / "frame_io.o")],
         obj / "pvvc_decode.o": [HIPCC, *HIP_FLAGS, f"-I{inc}", "-c", str(CSRC / "pvvc_decode.hip"), "-o",
                                 str(obj / "pvvc_decode.o")],
+        obj / "pvvc_serve.o": [HIPCC, *HIP_FLAGS, f"-I{inc}", "-c", str(CSRC / "pvvc_serve.hip"), "-o",
+                               str(obj / "pvvc_serve.o")],
         OUT / "libpvvc_enc.so": [CXX, *CXX_FLAGS, f"-I{inc}", "-shared", str(CSRC / "stream_writer.cpp"), "-o",
                                  str(OUT / "libpvvc_enc.so"), zlib, "-lpthread"],
     }
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(run, steps.values()))
     hip_so = OUT / "libpvvc_hip.so"
-    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(obj / "pvvc_decode.o"), str(obj / "frame_io.o"),
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(obj / "pvvc_decode.o"), str(obj / "pvvc_serve.o"),
+         str(obj / "frame_io.o"),
          "-o", str(hip_so), zlib, "-lpthread"])
     shutil.rmtree(obj, ignore_errors=True)
     return [hip_so, OUT / "libpvvc_enc.so"]

@@ -5,6 +5,7 @@
  */
 #include "pvvc_internal.h"
 #include "pvvc_kernels.h"
+#include "pvvc_runtime.h"
 
 #include <hip/hip_runtime.h>
 
@@ -489,7 +490,7 @@ __device__ __forceinline__ void emit_leaf(uint32_t c, float cx, float cy, float 
                                           const Sink& sink) {
     const bool deferred = predict && a.stage != nullptr;
     if (deferred) { /* reference added after the launch (finish_kernel) */
-        float* sp = a.stage + 3 * (a.out_off + c);
+        float* sp = a.stage + 3 * (static_cast<uint64_t>(a.stage_off) + c);
         sp[0] = Y;
         sp[1] = U;
         sp[2] = V;
@@ -898,6 +899,7 @@ __global__ __launch_bounds__(64 * WAVES) void dropin_kernel(Points* out, const i
     a.wait = 0;
     a.stage = nullptr;
     a.out_off = static_cast<uint64_t>(static_cast<int64_t>(inner_offset[s]));
+    a.stage_off = 0;
     const float* m = mv[s];
 #pragma unroll
     for (int i = 0; i < 12; ++i) a.mv[i] = m[i];
@@ -918,6 +920,7 @@ __device__ __forceinline__ SliceArgs item_args(const ItemDev* __restrict__ it) {
     a.qp = it->qp;
     a.wait = it->wait;
     a.out_off = it->out_off;
+    a.stage_off = it->stage_off;
     a.stage = nullptr;
 #pragma unroll
     for (int i = 0; i < 12; ++i) a.mv[i] = it->mv[i];
@@ -998,7 +1001,7 @@ __global__ __launch_bounds__(256) void finish_kernel(const ItemDev* __restrict__
     const uint32_t rs = slot->n_cap;
     for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
         const uint64_t o = it->out_off + c;
-        const float* sp = stage + 3 * o;
+        const float* sp = stage + 3 * (static_cast<uint64_t>(it->stage_off) + c);
         const float Y = sp[0] + ref[c];
         const float U = sp[1] + ref[rs + c];
         const float V = sp[2] + ref[2 * rs + c];
@@ -1180,35 +1183,6 @@ bool state_reset(StateInfo& si) {
 
 namespace pvvc {
 
-/* Queue segment: one persistent launch. */
-struct Segment {
-    uint32_t begin, count;       /* range in order[] */
-    uint32_t dbegin, dcount;     /* deferred predictive items, range in defer[] */
-    uint64_t ws_need;            /* per-item workspace bound */
-};
-
-/* Host-side plan of a batch image: [items][order][status][defer][frame blobs],
- * 256-B aligned sections.  Frame f's slices use state slots
- * slot_base[f] + index[s]. */
-struct BatchPlan {
-    std::vector<ItemDev> items;
-    std::vector<uint64_t> geo_at, col_at;  /* blob offsets in the image (UINT64_MAX: none) */
-    std::vector<uint32_t> order, defer;
-    std::vector<Segment> segs;
-    std::vector<int64_t> frame_off, frame_pts;
-    std::vector<uint32_t> frame_first, frame_count;
-    std::vector<size_t> frame_data_at;
-    uint32_t n_items = 0;
-    int64_t total_points = 0, input_bytes = 0, state_bytes = 0;
-    size_t o_items = 0, o_order = 0, o_status = 0, o_defer = 0, o_data = 0, total = 0;
-};
-
-/* Workspace bound of one item (coefficients, first-child and weight arrays,
- * top-down buffers); geo_bytes is the geometry blob it expands. */
-inline uint64_t item_ws_bound(uint32_t n, uint64_t geo_bytes) {
-    return 12ull * n + 2 * (4 * (geo_bytes + 1) + 16) + 48ull * n + 512;
-}
-
 /* Plans the batch of frames [0, n_frames) in decode order.
  *  - a predictive item whose patch's intra item is earlier in the same
  *    segment expands that item's geometry and is deferred (its reference is
@@ -1278,6 +1252,7 @@ int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_ba
             x.type = fr->type[s];
             x.qp = fr->qp[s];
             x.out_off = static_cast<uint64_t>(pts_acc + inner);
+            x.stage_off = static_cast<uint32_t>(pts_acc + inner);
             for (int k = 0; k < 12; ++k) x.mv[k] = fr->mv[16 * s + k];
             gcap_of[it] = geo_cap[f];
             if (x.type & PVVC_TYPE_PREDICT) {
@@ -1304,6 +1279,7 @@ int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_ba
         for (uint32_t s = 0; s < fr->slice_cnt; ++s) p.input_bytes += fr->geo_size[s] + fr->col_size[s];
     }
     p.total_points = pts_acc;
+    if (pts_acc >= (int64_t{1} << 32)) return PVVC_ERR_CAPACITY; /* stage offsets are 32-bit */
     p.order.assign(n_items, 0);
     seg_starts.push_back(n_items);
     for (size_t si = 0; si + 1 < seg_starts.size(); ++si) {
@@ -1399,20 +1375,17 @@ void launch_frame(const pvvc_batch* b, int f, const Sink& sink, uint32_t lds, hi
 }
 
 template <int W>
-void launch_persistent(pvvc_batch* b, const Segment& sg, const Sink& sink, uint32_t lds, uint32_t grid,
-                       uint32_t launch_id, hipStream_t s) {
-    pvvc_decoder* d = b->dec;
-    const uint32_t chunk = static_cast<uint32_t>(std::max(1, d->tuning.queue_chunk));
-    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, d->queue);
+void launch_persistent_w(const BatchDev& b, const Segment& sg, const Sink& sink, uint32_t lds, uint32_t grid,
+                         uint32_t chunk, const LaunchRes& r, uint32_t launch_id, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, r.queue);
     auto ev = g_ktiming.next();
     hipEventRecord(ev.first, s);
-    hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b->items, b->order + sg.begin, sg.count,
-                       d->queue, d->st.slots, sink, lds, d->slab, d->slab_bytes, launch_id, b->status, d->stage,
-                       chunk);
+    hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b.items, b.order + sg.begin, sg.count,
+                       r.queue, r.slots, sink, lds, r.slab, r.slab_bytes, launch_id, b.status, r.stage, chunk);
     hipEventRecord(ev.second, s);
     if (sg.dcount)
-        hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, b->items, b->defer + sg.dbegin, sg.dcount,
-                           d->st.slots, sink, d->stage, b->status);
+        hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, b.items, b.defer + sg.dbegin, sg.dcount,
+                           r.slots, sink, r.stage, b.status);
 }
 
 template <int W>
@@ -1429,6 +1402,55 @@ void set_all_lds_attrs(int bytes) {
 }
 
 }  // namespace
+
+namespace pvvc {
+
+bool rt_hip_ok(hipError_t e) { return hip_ok(e); }
+void rt_timing_reset() {
+    g_timing.reset();
+    g_ktiming.reset();
+}
+void rt_set_error(int e) { set_error(e); }
+
+void rt_init_kernels() {
+    static std::once_flag once;
+    std::call_once(once, [] { set_all_lds_attrs(clamp_lds(1 << 30)); });
+}
+
+bool rt_warm_kernels(const pvvc_tuning& t, uint32_t* queue, hipStream_t s) {
+    rt_init_kernels();
+    const uint32_t lds = static_cast<uint32_t>(clamp_lds(t.lds_bytes));
+    const Sink sink{nullptr, nullptr, nullptr, PVVC_OUT_NATIVE, 0};
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, queue);
+    if (t.waves_per_patch == 2)
+        hipLaunchKernelGGL(persistent_kernel<2>, dim3(1), dim3(128), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u);
+    else if (t.waves_per_patch == 4)
+        hipLaunchKernelGGL(persistent_kernel<4>, dim3(1), dim3(256), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u);
+    else
+        hipLaunchKernelGGL(persistent_kernel<1>, dim3(1), dim3(64), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u);
+    return hip_ok(hipGetLastError()) && hip_ok(hipStreamSynchronize(s));
+}
+
+uint32_t persistent_grid(const pvvc_tuning& t, uint32_t count) {
+    const uint32_t cap = static_cast<uint32_t>(std::max(1, devinfo().cu) * std::max(1, t.wgs_per_cu));
+    return std::max(1u, std::min(count, cap));
+}
+
+void launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, const pvvc_tuning& t, const LaunchRes& r,
+                    uint32_t launch_id, hipStream_t s) {
+    if (!sg.count) return;
+    const uint32_t lds = static_cast<uint32_t>(clamp_lds(t.lds_bytes));
+    const uint32_t grid = persistent_grid(t, sg.count);
+    const uint32_t chunk = static_cast<uint32_t>(std::max(1, t.queue_chunk));
+    if (t.waves_per_patch == 2) launch_persistent_w<2>(b, sg, sink, lds, grid, chunk, r, launch_id, s);
+    else if (t.waves_per_patch == 4) launch_persistent_w<4>(b, sg, sink, lds, grid, chunk, r, launch_id, s);
+    else launch_persistent_w<1>(b, sg, sink, lds, grid, chunk, r, launch_id, s);
+}
+
+}  // namespace pvvc
 
 extern "C" {
 
@@ -1846,13 +1868,11 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
         return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
     }
     if (mode != PVVC_MODE_PERSISTENT) return PVVC_ERR_INVALID_ARG;
-    DeviceInfo& di = devinfo();
-    const uint32_t cap_grid = static_cast<uint32_t>(std::max(1, di.cu) * std::max(1, d->tuning.wgs_per_cu));
     uint64_t need = 0;
     uint32_t grid_max = 0;
     for (auto& sg : b->segs) {
         need = std::max(need, sg.ws_need);
-        grid_max = std::max(grid_max, std::min(sg.count, cap_grid));
+        grid_max = std::max(grid_max, persistent_grid(d->tuning, sg.count));
     }
     if (need * grid_max > d->slab_total || need > d->slab_bytes) {
         if (d->slab) {
@@ -1880,15 +1900,14 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
         }
         d->stage_bytes = stage_need;
     }
+    const BatchDev bd{b->items, b->order, b->defer, b->status};
+    const LaunchRes res{d->st.slots, d->slab, d->slab_bytes, d->stage, d->queue};
     for (auto& sg : b->segs) {
         if (!sg.count) continue;
-        const uint32_t grid = std::min(sg.count, cap_grid);
         const uint32_t id = ++d->launch_id == 0 ? ++d->launch_id : d->launch_id;
         auto ev = g_timing.next();
         hipEventRecord(ev.first, g_stream);
-        if (W == 2) launch_persistent<2>(b, sg, sink, lds, grid, id, g_stream);
-        else if (W == 4) launch_persistent<4>(b, sg, sink, lds, grid, id, g_stream);
-        else launch_persistent<1>(b, sg, sink, lds, grid, id, g_stream);
+        launch_segment(bd, sg, sink, d->tuning, res, id, g_stream);
         hipEventRecord(ev.second, g_stream);
     }
     return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;

@@ -80,7 +80,7 @@ struct ItemDev {
     uint8_t wait;         /* predictive item whose intra item is in the same batch:
                              its reference colours are added after the launch */
     uint8_t pad0;
-    uint32_t pad1;
+    uint32_t stage_off;   /* first staged point of a deferred item (batch-relative) */
     uint64_t out_off;     /* first output point */
     float mv[12];         /* rows 0..2 of the row-major 4x4 */
 };
@@ -105,10 +105,11 @@ struct SliceArgs {
     uint32_t qp;
     uint32_t wait;
     uint64_t out_off;
+    uint32_t stage_off;
     float mv[12];
     float* stage;      /* predictive slice whose intra slice is decoded in the same
                           launch: unreferenced leaf YUV goes here (3 floats at
-                          3 * (out_off + leaf)); the reference is added by the
+                          3 * (stage_off + leaf)); the reference is added by the
                           finish pass after the launch */
 };
 

@@ -380,6 +380,104 @@ class Decoder:
 
 
 # ---------------------------------------------------------------------------
+# Serving loop
+# ---------------------------------------------------------------------------
+STREAM_WAITING, STREAM_ACTIVE, STREAM_CLOSING, STREAM_CLOSED = 0, 1, 2, 3
+
+
+class Server:
+    """Multi-stream decode server (continuous batching, HBM admission).
+
+    Frames passed to submit() are kept referenced here until their
+    completion has been polled."""
+
+    def __init__(self, hbm_budget: int = 0, lanes: int = 0, max_slots: int = 0, max_points_per_patch: int = 0,
+                 out_pool_points: int = 0, max_batch_points: int = 0, tuning: dict | None = None):
+        from ._native import ServerConfig
+        t = Tuning(**(tuning or {}))
+        cfg = ServerConfig(hbm_budget, lanes, max_slots, max_points_per_patch, out_pool_points, max_batch_points, t)
+        h = C.c_void_p()
+        _check(hip_lib().pvvc_server_create(C.byref(cfg), C.byref(h)), "pvvc_server_create")
+        self.handle = h
+        self._held: dict[tuple[int, int], Frame] = {}
+        self._seq: dict[int, int] = {}
+
+    @staticmethod
+    def clock() -> float:
+        return float(hip_lib().pvvc_server_clock())
+
+    def close(self) -> None:
+        if self.handle and self.handle.value:
+            hip_lib().pvvc_server_destroy(self.handle)
+            self.handle = C.c_void_p()
+        self._held.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def open_stream(self, max_patches: int, max_points_per_patch: int, max_points_per_frame: int,
+                    out_ring: int = 2) -> int:
+        sid = C.c_int()
+        _check(hip_lib().pvvc_server_open_stream(self.handle, max_patches, max_points_per_patch, max_points_per_frame,
+                                                 out_ring, C.byref(sid)), "pvvc_server_open_stream")
+        self._seq[sid.value] = 0
+        return sid.value
+
+    def stream_state(self, sid: int) -> int:
+        return int(hip_lib().pvvc_server_stream_state(self.handle, sid))
+
+    def submit(self, sid: int, frame: Frame, arrival: float | None = None, tag: int = 0) -> int:
+        """Queues a frame; returns its sequence number within the stream."""
+        if arrival is None:
+            arrival = self.clock()
+        _check(hip_lib().pvvc_server_submit(self.handle, sid, frame.handle, arrival, tag), "pvvc_server_submit")
+        seq = self._seq[sid]
+        self._seq[sid] = seq + 1
+        self._held[(sid, seq)] = frame
+        return seq
+
+    def close_stream(self, sid: int) -> None:
+        _check(hip_lib().pvvc_server_close_stream(self.handle, sid), "pvvc_server_close_stream")
+
+    def poll(self, max_n: int = 256, timeout_ms: int = 100) -> list[dict]:
+        from ._native import Completion
+        buf = (Completion * max_n)()
+        k = hip_lib().pvvc_server_poll(self.handle, buf, max_n, timeout_ms)
+        if k < 0:
+            raise PvvcError(-k, "pvvc_server_poll")
+        out = []
+        for c in buf[:k]:
+            self._held.pop((c.stream_id, c.seq), None)
+            out.append({f: getattr(c, f) for f, _ in Completion._fields_})
+        return out
+
+    def read(self, comp: dict) -> tuple[np.ndarray, np.ndarray]:
+        """Copies a completion's output to the host: (xyz f32[N,3], rgb u8[N,3])."""
+        n = int(comp["points"])
+        xyz = np.empty((n, 3), np.float32)
+        rgb = np.empty((n, 3), np.uint8)
+        if n:
+            _check(hip_lib().pvvc_memcpy_d2h(xyz.ctypes.data, C.c_void_p(comp["xyz"]), 12 * n), "d2h")
+            _check(hip_lib().pvvc_memcpy_d2h(rgb.ctypes.data, C.c_void_p(comp["rgb"]), 3 * n), "d2h")
+        return xyz, rgb
+
+    def release(self, comp: dict) -> None:
+        _check(hip_lib().pvvc_server_release(self.handle, comp["stream_id"], comp["out_slot"]), "release")
+
+    def stats(self) -> dict:
+        from ._native import ServerStats
+        st = ServerStats()
+        _check(hip_lib().pvvc_server_get_stats(self.handle, C.byref(st)), "pvvc_server_get_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self) -> None:
+        _check(hip_lib().pvvc_server_reset_stats(self.handle), "reset_stats")
+
+
+# ---------------------------------------------------------------------------
 # Synthetic streams (stream writer)
 # ---------------------------------------------------------------------------
 SYNTH_PRESETS = {

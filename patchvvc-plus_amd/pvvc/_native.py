@@ -36,6 +36,25 @@ class Tuning(C.Structure):
                 ("queue_chunk", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("lanes", C.c_int), ("max_slots", C.c_int),
+                ("max_points_per_patch", C.c_int), ("out_pool_points", C.c_int64), ("max_batch_points", C.c_int64),
+                ("tuning", Tuning)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("stream_id", C.c_int), ("status", C.c_int), ("seq", C.c_int64), ("tag", C.c_uint64),
+                ("points", C.c_int64), ("xyz", C.c_void_p), ("rgb", C.c_void_p), ("out_slot", C.c_int),
+                ("arrival_s", C.c_double), ("launch_s", C.c_double), ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("frames_done", C.c_int64), ("points_done", C.c_int64), ("batches", C.c_int64),
+                ("hbm_budget", C.c_uint64), ("hbm_used", C.c_uint64), ("streams_active", C.c_int),
+                ("streams_waiting", C.c_int), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("max_ms", C.c_double), ("mpts_per_s", C.c_double)]
+
+
 class SynthConfig(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("grid_bits", "n_frames", "gop", "qp_i", "qp_p", "n_humanoids", "jitter",
                                        "patch_min", "patch_max", "zstd_level", "threads")] + \
@@ -99,6 +118,17 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_batch_state_bytes", C.c_int64, c_vp)
     _sig(L, "pvvc_batch_status", C.c_int, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_decode_batch", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp)
+    _sig(L, "pvvc_server_clock", C.c_double)
+    _sig(L, "pvvc_server_create", C.c_int, C.POINTER(ServerConfig), C.POINTER(c_vp))
+    _sig(L, "pvvc_server_destroy", C.c_int, c_vp)
+    _sig(L, "pvvc_server_open_stream", C.c_int, c_vp, C.c_int, C.c_int, C.c_int64, C.c_int, c_i32p)
+    _sig(L, "pvvc_server_stream_state", C.c_int, c_vp, C.c_int)
+    _sig(L, "pvvc_server_submit", C.c_int, c_vp, C.c_int, c_vp, C.c_double, C.c_uint64)
+    _sig(L, "pvvc_server_close_stream", C.c_int, c_vp, C.c_int)
+    _sig(L, "pvvc_server_poll", C.c_int, c_vp, C.POINTER(Completion), C.c_int, C.c_int)
+    _sig(L, "pvvc_server_release", C.c_int, c_vp, C.c_int, C.c_int)
+    _sig(L, "pvvc_server_get_stats", C.c_int, c_vp, C.POINTER(ServerStats))
+    _sig(L, "pvvc_server_reset_stats", C.c_int, c_vp)
     _sig(L, "pvvc_dev_alloc", c_vp, C.c_size_t)
     _sig(L, "pvvc_dev_free", None, c_vp)
     _sig(L, "pvvc_memcpy_h2d", C.c_int, c_vp, c_vp, C.c_size_t)

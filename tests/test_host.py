@@ -150,3 +150,19 @@ def test_library_fails_loudly_without_device(pvvc):
         pytest.skip("a GPU is present")
     with pytest.raises(pvvc.PvvcError):
         pvvc.Decoder(4, 100)
+
+
+def test_poisson_schedule_properties():
+    """The serving driver's arrival schedule: sorted, per-stream frame numbers
+    consecutive from 0, mean gap close to 1/fps."""
+    import importlib
+    serve = importlib.import_module("pvvc.serve")
+    ev = serve.poisson_schedule(16, 30.0, 20.0, seed=5)
+    ts = [e[0] for e in ev]
+    assert ts == sorted(ts) and ts[0] >= 0 and ts[-1] < 20.0
+    per = {}
+    for t, s, k in ev:
+        assert k == per.get(s, -1) + 1
+        per[s] = k
+    rate = len(ev) / (16 * 20.0)
+    assert 27.0 < rate < 33.0
