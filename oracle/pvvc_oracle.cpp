@@ -190,6 +190,7 @@ struct RlgrDec {
     uint64_t buf = 0;
     int cnt = 0;
     bool overrun = false;
+    bool bad_run = false; /* run exponent k > 30: 1 << k would leave the symbol range */
     void fill() { /* :163-176 */
         while (cnt <= 56) {
             if (p != end) {
@@ -244,6 +245,7 @@ struct RlgrDec {
         buf = 0;
         cnt = 0;
         overrun = false;
+        bad_run = false;
         uint64_t u, kP = 0, kRP = 2 * L, m = 0, k, kR, q;
         int64_t n = 0;
         auto emit = [&](int64_t v) {
@@ -259,8 +261,13 @@ struct RlgrDec {
                     m += uint64_t{1} << k;
                     kP += U1;
                     k = kP / L;
+                    if (k > 30) {
+                        bad_run = true;
+                        break;
+                    }
                     if (overrun) break;
                 }
+                if (bad_run) break;
                 m += read(static_cast<int>(k));
                 if (overrun) break;
                 /* emit m zeros; a run past `size` only lengthens the tail the
@@ -585,9 +592,16 @@ struct PatchDecoder {
         std::vector<int64_t> sym(3 * static_cast<size_t>(sz));
         RlgrDec dec;
         dec.decode(cbytes.data(), static_cast<int64_t>(cbytes.size()), 3 * static_cast<int64_t>(sz), sym.data());
-        if (dec.overrun) return fail_intra(predict, ORC_ERR_BITSTREAM_OVERRUN);
+        /* The reference detects no bitstream faults: it raises only the
+         * structural errors below (tree, size), so those take precedence and
+         * a bitstream fault is reported when the structure is sound. */
+        int rlgr_st = ORC_OK;
         for (int64_t v : sym)
-            if (v > INT32_MAX || v < INT32_MIN) return fail_intra(predict, ORC_ERR_SYMBOL_RANGE);
+            if (v > INT32_MAX || v < INT32_MIN) {
+                rlgr_st = ORC_ERR_SYMBOL_RANGE;
+                break;
+            }
+        if (rlgr_st == ORC_OK && (dec.overrun || dec.bad_run)) rlgr_st = ORC_ERR_BITSTREAM_OVERRUN;
         coefs.assign(sz, YUV{});
         if (fp64) coefs64.assign(sz, YUV64{});
         for (uint32_t i = 0; i < sz; ++i) {
@@ -614,6 +628,7 @@ struct PatchDecoder {
         } else if (sz != cloud.size()) {
             return ORC_ERR_UNMATCHED_SIZE; /* InvertRAHT :264-266 */
         }
+        if (rlgr_st != ORC_OK) return fail_intra(predict, rlgr_st);
         type = t;
         invert_raht(predict);
         have_intra = true;
@@ -745,7 +760,7 @@ int64_t orc_rlgr_encode(const int64_t* data, int64_t n, uint8_t* out, int64_t ca
 int orc_rlgr_decode(const uint8_t* data, int64_t nbytes, int64_t nsym, int64_t* out) {
     RlgrDec d;
     d.decode(data, nbytes, nsym, out);
-    return d.overrun ? ORC_ERR_BITSTREAM_OVERRUN : ORC_OK;
+    return (d.overrun || d.bad_run) ? ORC_ERR_BITSTREAM_OVERRUN : ORC_OK;
 }
 
 /* Per-patch decoder handle. */

@@ -992,7 +992,7 @@ __global__ __launch_bounds__(256) void finish_kernel(const ItemDev* __restrict__
     const uint32_t sn = slot->n;
     int st = status[i];
     if (sn == 0) st = PVVC_ERR_EMPTY_REFERENCE; /* the intra slice failed */
-    else if (st == PVVC_OK && sn != n) st = PVVC_ERR_UNMATCHED_SIZE;
+    else if (sn != n) st = PVVC_ERR_UNMATCHED_SIZE;
     if (st != PVVC_OK) {
         if (threadIdx.x == 0) status[i] = st;
         return;

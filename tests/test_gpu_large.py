@@ -1,0 +1,72 @@
+"""GPU parity at the large configurations: c2 (11-bit, ~3M points/frame,
+patches 100-20000) and c4 (12-bit, 8 humanoids, ~5M points/frame, qp 1,
+patches up to ~50000).  Large patches exceed the LDS budget and stream
+their workspace through global memory; deep octrees and qp 1 give long RLGR
+runs.  Two frames (intra + predictive) per configuration."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", params=["c2", "c4"])
+def large(request, pvvc, oracle, tmp_path_factory):
+    preset = request.param
+    imgs = pvvc.synth_stream(preset, n_frames=2, seed=3, threads=16)
+    d = tmp_path_factory.mktemp(preset)
+    paths = []
+    for k, b in enumerate(imgs):
+        p = d / f"{preset}_{k}.frame"
+        p.write_bytes(b)
+        paths.append(p)
+    dec = oracle.StreamDecoder(fp64=False)
+    refs = [dec.decode(oracle.OracleFrame(str(p)), 16) for p in paths]
+    frames = [pvvc.Frame.load(str(p)) for p in paths]
+    sizes = np.concatenate([f.meta()["size"] for f in frames])
+    _say(preset, "points/frame", [f.points for f in frames], "patch sizes", int(sizes.min()), int(sizes.max()))
+    return preset, frames, refs
+
+
+def test_large_oracle_statuses_ok(large):
+    preset, frames, refs = large
+    for r in refs:
+        assert (r["status"] == 0).all()
+    if preset == "c4":
+        assert max(int(f.meta()["size"].max()) for f in frames) > 20000
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["per_frame", "persistent"])
+def test_large_native_matches_oracle(pvvc, large, mode):
+    preset, frames, refs = large
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames)
+    _say(preset, "native mode", mode)
+    dec = pvvc.Decoder(n_patches, max_pts)
+    b = dec.batch(frames)
+    xyz, rgb = dec.decode_host(b, mode, pvvc.OUT_NATIVE)
+    assert (b.status() == 0).all(), np.unique(b.status(), return_counts=True)
+    for k, ref in enumerate(refs):
+        o, n = b.frame_offset[k], b.frame_points[k]
+        np.testing.assert_array_equal(xyz[o:o + n], ref["xyz"])
+        np.testing.assert_array_equal(rgb[o:o + n], ref["rgb"])
+
+
+def test_large_dropin_matches_oracle(pvvc, large):
+    preset, frames, refs = large
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames)
+    st = pvvc.State(n_patches, max_pts)
+    for k, (f, ref) in enumerate(zip(frames, refs)):
+        _say(preset, "drop-in frame", k)
+        pts = pvvc.decode_dropin(f, st)
+        xyz = np.stack([pts["x"], pts["y"], pts["z"]], 1)
+        np.testing.assert_array_equal(xyz, ref["xyz"])
+        rgbf = np.stack([pts["r"], pts["g"], pts["b"]], 1)
+        np.testing.assert_array_equal(rgbf, ref["rgbf"] / np.float32(255))

@@ -6,25 +6,9 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+from _frames import pack_frame as _pack_frame
 
 REPO = Path(__file__).resolve().parent.parent
-
-
-def _pack_frame(ts, slices):
-    """Hand-packed .frame image, field by field in the order of
-    ChangeSliceToFrame (src/io/slice_io.cc:258-315)."""
-    S = len(slices)
-    out = struct.pack("<iI", ts, S)
-    out += b"".join(struct.pack("<i", s["index"]) for s in slices)
-    out += bytes(s["type"] for s in slices)
-    out += b"".join(struct.pack("<I", s["size"]) for s in slices)
-    out += bytes(s["qp"] for s in slices)
-    out += b"".join(struct.pack("<16f", *s["mv"]) for s in slices)
-    out += b"".join(struct.pack("<I", len(s["geo"])) for s in slices)
-    out += b"".join(s["geo"] for s in slices)
-    out += b"".join(struct.pack("<I", len(s["col"])) for s in slices)
-    out += b"".join(s["col"] for s in slices)
-    return out
 
 
 def _zstd(data: bytes) -> bytes:
