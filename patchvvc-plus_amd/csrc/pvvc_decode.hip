@@ -130,165 +130,192 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
     return static_cast<int32_t>(h ^ (0u - static_cast<uint32_t>(u & 1u)));
 }
 
-/* Common-case RLGR symbols as a tight scalar loop (entropy_codec.cc:207-283):
- *  - window refill from the VGPR chunk `cur` (v_readlane), 32 bits at a time;
+/* Common-case RLGR symbols as a tight loop on the scalar unit
+ * (entropy_codec.cc:207-283).  State in SGPRs: the 64-bit window `win`
+ * (left aligned, zero below its nb valid bits) with nbx = nb - 33, the next
+ * dword `wpos` of the chunk held one dword per lane in `cur`, k_RP, k_P and
+ * neg = nrel - lim (mod 2^32; its carry out marks the end of the 64-symbol
+ * chunk).  Per symbol:
  *  - k = 0: Golomb-Rice code (unary prefix by one leading-ones count,
  *    remainder by shifts), k_P / k_RP adaptation (:255-276);
- *  - k > 0 with no 1-bits in the run prefix: the run length in k bits, the
- *    zero run (only advances the symbol index), then the Golomb-Rice coded
- *    symbol + 1 and the adaptation of :240-251;
- *  - each decoded symbol is written into lane `nrel` of `acc`.
- * Every case is checked before any bit is consumed; the loop returns to the
- * caller, with the state exactly at a symbol boundary, on: chunk end
- * (nrel == lim), a chunk rotation of `cur`, an escape or a unary prefix of 32+,
- * k_R > 24, a code not entirely inside the 64-bit window, 1-bits in a run
- * prefix, or a zero run reaching the chunk end.  s80-s95 are scratch; no
- * memory operations. */
-__device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nb, uint32_t& wpos, uint32_t& kRP, uint32_t& kP,
-                                              uint32_t& nrel, uint32_t lim, uint32_t cur, int32_t& acc, uint32_t lane) {
+ *  - k > 0: up to 6 one-bits of the run prefix (each adds 2^k, k_P += 2),
+ *    the run length in k bits, the zero run (only advances the symbol index),
+ *    then the Golomb-Rice coded symbol + 1 and the adaptation of :240-251;
+ *  - the raw Golomb value u is written into lane `nrel` of `acc`
+ *    (Unsign2Sign is applied when the chunk is stored);
+ *  - the window takes 32 more bits when fewer than 33 remain; the borrow of
+ *    nbx -= length says so.
+ * A code is decoded here only when it lies entirely in the first 32 bits of
+ * the window, so every check precedes any state change.  The loop returns
+ * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
+ * of `cur`, a unary prefix of 32+ or a code longer than 32 bits, more than 6
+ * one-bits in a run prefix or k > 24 after them, or a zero run reaching the
+ * chunk end.  s80-s93 are scratch.  One SGPR written by a scalar
+ * instruction and then read as a v_readlane lane select needs 4 wait states
+ * (here: four instructions and an s_nop 1). */
+__device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint32_t& wpos, uint32_t& kRP,
+                                               uint32_t& kP, uint32_t& neg, uint32_t cur, int32_t& acc,
+                                               uint32_t vrev) {
     int32_t vt;
     asm volatile(
+        /* ================= symbol start ================= */
         "1:\n\t"
-        "s_cmp_ge_u32 %[nrel], %[lim]\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc1 30f\n\t"
+        /* ---- k = 0: GR(k_R) ---- */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"        /* ones; -1 if the window is all ones */
+        "s_add_u32 s86, s84, 1\n\t"                /* carry iff ones == -1 */
         "s_cbranch_scc1 9f\n\t"
-        /* refill: 32 more bits when at most 32 remain */
-        "s_cmp_gt_i32 %[nb], 32\n\t"
-        "s_cbranch_scc1 2f\n\t"
+        "s_lshr_b32 s85, %[kRP], 2\n\t"            /* k_R */
+        "s_add_u32 s87, s86, s85\n\t"              /* code length */
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 9f\n\t"                    /* escape / long code: C++ */
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"     /* drop the ones: [0][rem]... */
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"   /* rem (k_R bits) */
+        "s_lshl_b64 %[win], %[win], s87\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"             /* ones << k_R; SCC = ones != 0 */
+        "s_cselect_b32 s90, -1, -2\n\t"            /* k_RP step: ones ? ones - 1 : -2 */
+        "s_or_b32 s89, s89, s82\n\t"               /* u; SCC = u != 0 */
+        "s_cselect_b32 s88, -1, 3\n\t"
+        "s_add_i32 %[kP], %[kP], s88\n\t"
+        "s_max_i32 %[kP], %[kP], 0\n\t"            /* k_P: u ? max(k_P - 1, 0) : k_P + 3 */
+        "s_add_i32 s88, %[kRP], s84\n\t"
+        "s_add_i32 s88, s88, s90\n\t"
+        "s_max_i32 s88, s88, 0\n\t"
+        "s_min_i32 %[kRP], s88, 128\n\t"
+        "v_mov_b32 %[vt], s89\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"        /* borrow iff nb < 33 after the code */
+        "s_cbranch_scc1 20f\n\t"
+        "10:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"          /* carry iff the chunk is complete */
+        "s_cbranch_scc0 1b\n\t"
+        "s_branch 9f\n\t"
+        /* refill (nb <= 32): 32 bits from dword wpos of the chunk */
+        "20:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 9f\n\t"                   /* next chunk needed */
-        "s_nop 3\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_cbranch_scc1 21f\n\t"                   /* next chunk needed: C++ */
         "s_mov_b32 s91, 0\n\t"
-        "s_sub_u32 s89, 32, %[nb]\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"            /* 32 - nb */
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
-        "s_add_i32 %[nb], %[nb], 32\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
-        "2:\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc1 3f\n\t"
-        /* ---- k = 0 ---- */
+        "s_branch 10b\n\t"
+        "21:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_branch 9f\n\t"
+        /* ---- k > 0: run (1-bits add 2^k each), m zeros, then GR(k_R) + 1 ---- */
+        "30:\n\t"
         "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"      /* ones; -1 when the window is all ones */
-        "s_cmp_gt_u32 s84, 31\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_lshr_b32 s85, %[kRP], 2\n\t"           /* k_R */
-        "s_cmp_gt_u32 s85, 24\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_add_u32 s86, s84, 1\n\t"
-        "s_add_u32 s87, s86, s85\n\t"             /* code length */
-        "s_cmp_gt_i32 s87, %[nb]\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s86\n\t"
-        "s_lshl_b64 %[win], s[80:81], s85\n\t"
-        "s_sub_i32 %[nb], %[nb], s87\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], 1\n\t"
-        "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[82:83], s88\n\t" /* remainder */
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_or_b32 s89, s89, s82\n\t"              /* u */
-        "s_cmp_eq_u32 s89, 0\n\t"
-        "s_cselect_b32 s88, 3, -1\n\t"
-        "s_add_i32 %[kP], %[kP], s88\n\t"
-        "s_max_i32 %[kP], %[kP], 0\n\t"           /* k_P: u ? max(k_P-1,0) : k_P+3 */
-        "s_branch 5f\n\t"
-        /* ---- k > 0: run, then symbol + 1 ---- */
-        "3:\n\t"
-        "s_lshr_b32 s85, %[kP], 2\n\t"            /* k */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"        /* t = 1-bits of the run prefix */
+        "s_mov_b32 s82, 0\n\t"                     /* m */
+        "s_mov_b32 s87, %[kP]\n\t"                 /* k_P while the run is read */
         "s_cmp_lg_u32 s84, 0\n\t"
-        "s_cbranch_scc1 9f\n\t"                   /* 1-bits in the run prefix */
-        "s_lshl_b64 s[80:81], %[win], 1\n\t"
+        "s_cbranch_scc1 40f\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"               /* k */
+        "31:\n\t"
+        "s_add_u32 s86, s84, 1\n\t"                /* prefix length */
+        "s_lshl_b64 s[80:81], %[win], s86\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t" /* m = next k bits */
-        "s_lshl_b64 s[92:93], s[80:81], s85\n\t" /* window after the run code */
-        "s_add_u32 s87, s85, 1\n\t"               /* run code length */
-        "s_add_u32 s86, %[nrel], s82\n\t"         /* index after the zeros */
-        "s_cmp_ge_u32 s86, %[lim]\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"   /* Read(k), k >= 1 */
+        "s_add_u32 s82, s82, s90\n\t"              /* m */
+        "s_add_u32 s83, %[neg], s82\n\t"           /* carry iff the zeros reach the chunk end */
         "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"   /* after the run code */
+        "s_add_u32 s86, s86, s85\n\t"              /* run code length */
         "s_not_b64 s[80:81], s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_cmp_gt_u32 s84, 31\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"        /* GR prefix ones */
+        "s_lshr_b32 s89, %[kRP], 2\n\t"            /* k_R */
+        "s_add_u32 s86, s86, s84\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_add_u32 s86, s86, 1\n\t"                /* total length */
+        "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 9f\n\t"
-        "s_lshr_b32 s85, %[kRP], 2\n\t"           /* k_R */
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"   /* rem */
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"             /* SCC = ones != 0 */
+        "s_cselect_b32 s91, -1, -2\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"                /* u + 1 */
+        "s_sub_u32 %[kP], s87, 1\n\t"              /* k_P >= 4 here */
+        "s_add_i32 s90, %[kRP], s84\n\t"
+        "s_add_i32 s90, s90, s91\n\t"
+        "s_max_i32 s90, s90, 0\n\t"
+        "s_min_i32 %[kRP], s90, 128\n\t"
+        "s_mov_b32 %[neg], s83\n\t"
+        "v_mov_b32 %[vt], s88\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 20b\n\t"
+        "s_branch 10b\n\t"
+        /* 1-bits in the run prefix: up to 6, k stays <= 24 */
+        "40:\n\t"
+        "s_cmp_gt_u32 s84, 6\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_sub_u32 s83, 0, s84\n\t"
+        "41:\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"
+        "s_lshl_b32 s89, 1, s85\n\t"
+        "s_add_u32 s82, s82, s89\n\t"
+        "s_add_u32 s87, s87, 2\n\t"
+        "s_add_u32 s83, s83, 1\n\t"
+        "s_cbranch_scc0 41b\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"
         "s_cmp_gt_u32 s85, 24\n\t"
         "s_cbranch_scc1 9f\n\t"
-        "s_add_u32 s94, s84, 1\n\t"
-        "s_add_u32 s95, s94, s85\n\t"
-        "s_add_u32 s95, s95, s87\n\t"             /* run code + symbol code */
-        "s_cmp_gt_i32 s95, %[nb]\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s94\n\t"
-        "s_lshl_b64 %[win], s[80:81], s85\n\t"
-        "s_sub_i32 %[nb], %[nb], s95\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], 1\n\t"
-        "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[82:83], s88\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "s_add_u32 s89, s89, 1\n\t"               /* u + 1 */
-        "s_sub_i32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], 0\n\t"
-        "s_mov_b32 %[nrel], s86\n\t"
-        /* ---- common tail: k_RP, symbol, lane write ---- */
-        "5:\n\t"
-        "s_add_i32 s88, %[kRP], s84\n\t"
-        "s_add_i32 s88, s88, -1\n\t"
-        "s_min_u32 s88, s88, 128\n\t"
-        "s_sub_i32 %[kRP], %[kRP], 2\n\t"
-        "s_max_i32 %[kRP], %[kRP], 0\n\t"
-        "s_cmp_eq_u32 s84, 0\n\t"
-        "s_cselect_b32 %[kRP], %[kRP], s88\n\t"  /* k_RP: p ? min(k_RP+p-1,128) : max(k_RP-2,0) */
-        "s_lshr_b32 s88, s89, 1\n\t"
-        "s_bfe_i32 s89, s89, 0x10000\n\t"
-        "s_xor_b32 s88, s88, s89\n\t"             /* Unsign2Sign */
-        "v_mov_b32 %[vt], s88\n\t"
-        "v_cmp_eq_u32 vcc, %[nrel], %[lane]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
-        "s_add_u32 %[nrel], %[nrel], 1\n\t"
-        "s_branch 1b\n\t"
+        "s_branch 31b\n\t"
         "9:\n\t"
-        : [win] "+s"(win), [nb] "+s"(nb), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [nrel] "+s"(nrel),
+        : [win] "+s"(win), [nbx] "+s"(nbx), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [neg] "+s"(neg),
           [acc] "+v"(acc), [vt] "=&v"(vt)
-        : [lim] "s"(lim), [cur] "v"(cur), [lane] "v"(lane)
+        : [cur] "v"(cur), [vrev] "v"(vrev)
         : "vcc", "scc", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92",
-          "s93", "s94", "s95");
+          "s93");
 }
 
-/* Decodes 3n symbols into coef[0, 3n) (planar Y, U, V) dequantised with qp.
- * Called by all 64 lanes of one wavefront; control flow is wave-uniform.
- * Symbols are gathered 64 at a time in `acc` (lane j = symbol base + j);
- * zero runs only advance the symbol index. */
 __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src, uint32_t nbytes, uint32_t total,
-                                                uint32_t qp, float* coef) {
+                                                 uint32_t qp, float* coef) {
     BitReader br;
     br_init(br, src, nbytes);
     br_refill(br);
     br_refill(br);
     const uint32_t lane = lane_id();
-    int32_t acc = 0;
+    int32_t acc = 0; /* raw u of symbol acc_base + lane */
     uint32_t acc_base = 0, nrel = 0;
     uint32_t kP = 0, kRP = 8;
     int st = PVVC_OK;
 
     auto store = [&](uint32_t base, int32_t v) {
         const uint32_t i = base + lane;
-        if (i < total) coef[i] = static_cast<float>(static_cast<long long>(v) * static_cast<long long>(qp));
+        const int32_t s = static_cast<int32_t>(unsign2sign(static_cast<uint32_t>(v)));
+        if (i < total) coef[i] = static_cast<float>(static_cast<long long>(s) * static_cast<long long>(qp));
     };
-    auto emit = [&](int32_t s) {
-        acc = (lane == nrel) ? s : acc;
+    auto emit = [&](uint32_t u) {
+        acc = (lane == nrel) ? static_cast<int32_t>(u) : acc;
         ++nrel;
     };
 
     while (acc_base < total) {
         const uint32_t rest = total - acc_base;
         const uint32_t lim = rest < 64u ? rest : 64u;
-        /* the adaptation state is wave-uniform; say so for the SGPR operands */
-        kRP = __builtin_amdgcn_readfirstlane(kRP);
-        kP = __builtin_amdgcn_readfirstlane(kP);
-        rlgr_fast_run(br.win, br.nb, br.wpos, kRP, kP, nrel, lim, br.cur, acc, lane);
+        if (nrel < lim && br.nb > 32) {
+            kRP = __builtin_amdgcn_readfirstlane(kRP);
+            kP = __builtin_amdgcn_readfirstlane(kP);
+            uint32_t neg = __builtin_amdgcn_readfirstlane(nrel - lim);
+            int32_t nbx = __builtin_amdgcn_readfirstlane(br.nb - 33);
+            rlgr_fast_run(br.win, nbx, br.wpos, kRP, kP, neg, br.cur, acc, lane - lim);
+            nrel = neg + lim;
+            br.nb = nbx + 33;
+        }
         if (nrel >= lim) { /* chunk of 64 symbols complete */
             store(acc_base, acc);
             acc = 0;
@@ -302,18 +329,16 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
         }
         const uint32_t kR = kRP >> 2;
         if (kP < 4) {
-            /* k = 0 symbol the fast loop declined: escape, long prefix, large k_R */
             const uint64_t u = br_gr(br, kR);
             if (u >= (1ull << 32)) {
                 st = PVVC_ERR_SYMBOL_RANGE;
                 break;
             }
-            emit(unsign2sign(u));
+            emit(static_cast<uint32_t>(u));
             kRP = adapt_krp(kRP, u >> kR);
             kP = u ? (kP ? kP - 1 : 0u) : kP + 3;
             continue;
         }
-        /* run-length coding (entropy_codec.cc:221-252) */
         uint32_t k = kP >> 2;
         uint64_t m = 0;
         bool bad = false;
@@ -342,7 +367,7 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
         }
         m += br_get(br, k);
         const uint64_t n = static_cast<uint64_t>(acc_base) + nrel;
-        if (m >= total - n) { /* the run reaches the end */
+        if (m >= total - n) {
             store(acc_base, acc);
             acc = 0;
             acc_base += 64u;
@@ -350,7 +375,7 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
             break;
         }
         const uint32_t n2 = static_cast<uint32_t>(n + m);
-        if (n2 - acc_base >= 64u) { /* the run leaves this chunk */
+        if (n2 - acc_base >= 64u) {
             store(acc_base, acc);
             acc = 0;
             const uint32_t nb2 = n2 & ~63u;
@@ -363,15 +388,14 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
             st = PVVC_ERR_SYMBOL_RANGE;
             break;
         }
-        emit(unsign2sign(u + 1));
+        emit(static_cast<uint32_t>(u + 1));
         kRP = adapt_krp(kRP, u >> kR);
         kP = kP ? kP - 1 : 0u;
     }
-    /* the chunk in flight and everything after it */
     if (acc_base < total) store(acc_base, acc);
     for (uint32_t b = acc_base + 64u; b < total; b += 64u) store(b, 0);
-    const uint64_t consumed = static_cast<uint64_t>(br.wpos) * 32u - static_cast<uint64_t>(br.nb);
-    if (st == PVVC_OK && consumed > static_cast<uint64_t>(nbytes) * 8u) st = PVVC_ERR_BITSTREAM;
+    const int64_t consumed = static_cast<int64_t>(br.wpos) * 32 - br.nb;
+    if (st == PVVC_OK && consumed > 8ll * nbytes) st = PVVC_ERR_BITSTREAM;
     return st;
 }
 
