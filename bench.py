@@ -161,6 +161,7 @@ def main() -> int:
     ap.add_argument("--lds", type=int, default=0)
     ap.add_argument("--wgs-per-cu", type=int, default=0)
     ap.add_argument("--queue-chunk", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true", help="run the decoder autotuner on the first GoP before timing")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
     ap.add_argument("--no-serving", action="store_true", help="skip the multi-stream serving leg")
     ap.add_argument("--serve-streams", type=int, default=64)
@@ -219,6 +220,12 @@ def main() -> int:
         dec.set_tuning(**tun)
     batches = [dec.batch(frames[g * gop:(g + 1) * gop]) for g in range(args.gops)]
     outs = [dec.outputs(b, pvvc.OUT_NATIVE) for b in batches]
+    autotuned = None
+    if args.autotune and run_b:
+        t_at = time.time()
+        best, best_ms = dec.autotune(batches[0], pvvc.MODE_PERSISTENT, outs[0], reps=3)
+        autotuned = {"tuning": best, "ms": round(best_ms, 4), "seconds": round(time.time() - t_at, 1)}
+        log("autotune", autotuned)
     for i in range(args.warmup if run_b else 0):
         dec.decode(batches[i % args.gops], pvvc.MODE_PERSISTENT, outs[i % args.gops])
     pvvc.synchronize()
@@ -367,7 +374,7 @@ def main() -> int:
                                f"{int(np.mean(pts_per_frame))} pts/frame, {len(metas[0]['size'])} patches/frame",
                    "frames_per_step": gop, "points_per_step": int(np.mean(step_pts)),
                    "patches_per_frame": int(np.mean([len(m['size']) for m in metas])),
-                   "tuning": dec.tuning()},
+                   "tuning": dec.tuning(), "autotuned": autotuned},
         "frames_per_s": round(batched_fps, 1),
         "per_frame": {"frames_per_s": round(pf_fps, 1), "Mpts_per_s": round(pf_mpts, 2),
                       "kernel_ms_mean": round(float(np.mean(frame_kernel_ms)), 4),
