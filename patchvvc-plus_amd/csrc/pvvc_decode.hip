@@ -591,6 +591,25 @@ __device__ uint64_t* g_prof = nullptr;
  * coefficients in stream order).  Writes the 8 child DCs (meaningful for
  * occupied children).  FP32 in the reference's operation order; where one
  * side of a pair is empty the transform is the identity on the other side. */
+/* Haar butterfly factors for small weights: entry (w0 << 6) | w1 holds
+ * (sqrtf(w0) / sqrtf(w0 + w1), sqrtf(w1) / sqrtf(w0 + w1)) for 1 <= w0, w1 < 64,
+ * written once by raht_table_kernel with the very operations node_inverse
+ * uses for larger weights, so a lookup equals the computation bit for bit. */
+constexpr uint32_t kRahtTab = 64;
+__device__ float2 g_raht_tab[kRahtTab * kRahtTab];
+
+__device__ __forceinline__ float2 haar_factors(uint32_t w0, uint32_t w1) {
+    const float base = sqrtf(static_cast<float>(w0 + w1));
+    return make_float2(sqrtf(static_cast<float>(w0)) / base, sqrtf(static_cast<float>(w1)) / base);
+}
+
+__global__ void raht_table_kernel() {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kRahtTab * kRahtTab) return;
+    const uint32_t w0 = i / kRahtTab, w1 = i % kRahtTab;
+    g_raht_tab[i] = (w0 && w1) ? haar_factors(w0, w1) : make_float2(0.0f, 0.0f);
+}
+
 __device__ __forceinline__ void node_inverse(const uint32_t cw[8], float dY, float dU, float dV, const float* coef,
                                              uint32_t n, uint32_t r, float oY[8], float oU[8], float oV[8]) {
     uint32_t w[16];
@@ -614,9 +633,8 @@ __device__ __forceinline__ void node_inverse(const uint32_t cw[8], float dY, flo
         if (w0 != 0 && w1 != 0) {
             const float hY = coef[n - 1 - r], hU = coef[2 * n - 1 - r], hV = coef[3 * n - 1 - r];
             ++r;
-            const float base = sqrtf(static_cast<float>(w0 + w1));
-            const float ta = sqrtf(static_cast<float>(w0)) / base;
-            const float tc = sqrtf(static_cast<float>(w1)) / base;
+            const float2 t = (w0 < kRahtTab && w1 < kRahtTab) ? g_raht_tab[w0 * kRahtTab + w1] : haar_factors(w0, w1);
+            const float ta = t.x, tc = t.y;
             cY[ia] = cY[idx] * ta + hY * (-tc);
             cY[ib] = cY[idx] * tc + hY * ta;
             cU[ia] = cU[idx] * ta + hU * (-tc);
@@ -1356,7 +1374,7 @@ void fill_batch_image(const BatchPlan& p, const pvvc_frame* const* frames, uint8
 /* ---- native decoder / batch ---- */
 struct pvvc_decoder {
     StateInfo st;
-    pvvc_tuning tuning{1, 16 * 1024, 8, 1};
+    pvvc_tuning tuning{1, 8 * 1024, 16, 1};
     uint32_t launch_id = 0;
     uint8_t* slab = nullptr;
     uint64_t slab_bytes = 0, slab_total = 0;
@@ -1438,7 +1456,13 @@ void rt_set_error(int e) { set_error(e); }
 
 void rt_init_kernels() {
     static std::once_flag once;
-    std::call_once(once, [] { set_all_lds_attrs(clamp_lds(1 << 30)); });
+    std::call_once(once, [] {
+        set_all_lds_attrs(clamp_lds(1 << 30));
+        /* the Haar factor table, complete before any decode kernel runs */
+        hipLaunchKernelGGL(raht_table_kernel, dim3((kRahtTab * kRahtTab + 255) / 256), dim3(256), 0, nullptr);
+        hip_ok(hipGetLastError());
+        hip_ok(hipDeviceSynchronize());
+    });
 }
 
 bool rt_warm_kernels(const pvvc_tuning& t, uint32_t* queue, hipStream_t s) {
@@ -1622,8 +1646,7 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
         si = it->second;
     }
     const int lds = clamp_lds(g_dropin_tuning.lds_bytes);
-    static std::once_flag once;
-    std::call_once(once, [] { set_all_lds_attrs(clamp_lds(1 << 30)); });
+    rt_init_kernels();
     g_timing.reset();
     g_ktiming.reset();
     auto ev = g_timing.next();
@@ -1841,8 +1864,7 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
     if (!d || !b || b->dec != d) return PVVC_ERR_INVALID_ARG;
     if (output == PVVC_OUT_NATIVE && (!xyz || !rgb)) return PVVC_ERR_INVALID_ARG;
     if (output == PVVC_OUT_POINTS && !points) return PVVC_ERR_INVALID_ARG;
-    static std::once_flag once;
-    std::call_once(once, [] { set_all_lds_attrs(clamp_lds(1 << 30)); });
+    rt_init_kernels();
     Sink sink{xyz, rgb, points, output, g_white_small};
     const int W = d->tuning.waves_per_patch;
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(d->tuning.lds_bytes));
@@ -1950,7 +1972,7 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
     reps = std::max(1, std::min(reps, 50));
     static const int kWaves[] = {1, 2, 4};
     static const int kLds[] = {4096, 8192, 16384, 24576, 32768, 49152, 65536};
-    static const int kWgs[] = {2, 4, 6, 8, 12, 16};
+    static const int kWgs[] = {2, 4, 8, 12, 16, 20, 24, 32};
     static const int kChunk[] = {1, 2, 4, 8};
     const bool persistent = mode == PVVC_MODE_PERSISTENT;
     std::map<std::tuple<int, int, int, int>, double> seen;
@@ -1985,7 +2007,7 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
         for (int dim = 0; dim < 4; ++dim) {
             if (!persistent && dim >= 2) break; /* grid shape is the slice count there */
             const int* vals = dim == 0 ? kWaves : dim == 1 ? kLds : dim == 2 ? kWgs : kChunk;
-            const int nv = dim == 0 ? 3 : dim == 1 ? 7 : dim == 2 ? 6 : 4;
+            const int nv = dim == 0 ? 3 : dim == 1 ? 7 : dim == 2 ? 8 : 4;
             for (int k = 0; k < nv; ++k) {
                 pvvc_tuning t = cur;
                 int* field = dim == 0 ? &t.waves_per_patch : dim == 1 ? &t.lds_bytes : dim == 2 ? &t.wgs_per_cu

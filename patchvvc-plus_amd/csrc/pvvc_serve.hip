@@ -512,8 +512,8 @@ int pvvc_server_create(const pvvc_server_config* cfg_in, pvvc_server** out) {
     if (cfg.out_pool_points <= 0) cfg.out_pool_points = int64_t{64} << 20;
     if (cfg.max_batch_points <= 0) cfg.max_batch_points = int64_t{16} << 20;
     if (cfg.tuning.waves_per_patch <= 0) cfg.tuning.waves_per_patch = 1;
-    if (cfg.tuning.lds_bytes <= 0) cfg.tuning.lds_bytes = 16 * 1024;
-    if (cfg.tuning.wgs_per_cu <= 0) cfg.tuning.wgs_per_cu = 8;
+    if (cfg.tuning.lds_bytes <= 0) cfg.tuning.lds_bytes = 8 * 1024;
+    if (cfg.tuning.wgs_per_cu <= 0) cfg.tuning.wgs_per_cu = 16;
     if (cfg.tuning.queue_chunk <= 0) cfg.tuning.queue_chunk = 1;
     if (cfg.lanes > 8 || cfg.max_batch_points >= (int64_t{1} << 31)) return PVVC_ERR_INVALID_ARG;
     auto* sv = new pvvc_server();
