@@ -131,7 +131,8 @@ def serving_leg(args, pvvc) -> dict:
     st["sustained"] = bool(st["frames_completed"] == st["frames_offered"] and st["frames_failed"] == 0
                            and st["p99_ms"] <= 33.0)
     for k in ("p50_ms", "p99_ms", "max_ms", "mpts_per_s", "offered_mpts_per_s", "submit_late_p99_ms",
-              "queue_wait_p50_ms", "queue_wait_p99_ms", "service_p50_ms", "service_p99_ms", "frames_per_batch"):
+              "queue_wait_p50_ms", "queue_wait_p99_ms", "service_p50_ms", "service_p99_ms", "frames_per_batch",
+              "host_ms_per_batch"):
         st[k] = round(float(st[k]), 3)
     log("serving", st)
     return st

@@ -219,11 +219,14 @@ typedef struct pvvc_server_config {
     uint64_t hbm_budget;        /* bytes for admitted streams' state + output rings
                                    (0: 80% of the free HBM after the fixed buffers) */
     int lanes;                  /* batches in flight at once, each on its own HIP stream (default 3) */
-    int max_slots;              /* patch state slots over all streams (default 65536) */
+    int max_slots;              /* patch state slots over all streams (default 262144) */
     int max_points_per_patch;   /* largest patch any stream may send (default 4096) */
-    int64_t out_pool_points;    /* output pool shared by the streams' rings (default 64 Mi points) */
+    int64_t out_pool_points;    /* output pool shared by the streams' rings (default 256 Mi points,
+                                   15 B each) */
     int64_t max_batch_points;   /* points per batch (default 16 Mi) */
     pvvc_tuning tuning;         /* launch shape of the persistent decode */
+    int lanes_spin;             /* 1: completion threads spin on their lane's event (lowest wake-up
+                                   latency, one host core each); 0: they block */
 } pvvc_server_config;
 
 typedef struct pvvc_completion {
@@ -244,6 +247,8 @@ typedef struct pvvc_server_stats {
     int streams_active, streams_waiting;
     double p50_ms, p99_ms, max_ms;  /* arrival -> completion, per frame */
     double mpts_per_s;              /* points done / (last completion - first arrival) */
+    double host_ms_per_batch;       /* scheduler time to plan, pack and launch a batch */
+    int64_t backlog_frames;         /* frames queued, not yet in a batch */
 } pvvc_server_stats;
 
 /* Monotonic clock (seconds) for arrival stamps. */

@@ -1257,8 +1257,20 @@ int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_ba
     p.geo_at.assign(n_items, UINT64_MAX);
     p.col_at.assign(n_items, UINT64_MAX);
     std::vector<uint32_t> gcap_of(n_items, 0);
-    std::map<int, uint32_t> intra_item;      /* slot -> intra item in this segment */
-    std::map<int, int> seen;                 /* slots with items in this segment */
+    /* per-slot tables, valid where their stamp equals the current segment's */
+    int max_slot = 0;
+    for (int f = 0; f < n_frames; ++f) max_slot = std::max(max_slot, slot_base[f] + slot_count[f]);
+    thread_local std::vector<uint32_t> intra_at, intra_stamp, seen_stamp, pmax, pmax_stamp;
+    thread_local uint32_t epoch = 0;
+    if (intra_at.size() < static_cast<size_t>(max_slot)) {
+        intra_at.assign(max_slot, 0);
+        intra_stamp.assign(max_slot, 0);
+        seen_stamp.assign(max_slot, 0);
+        pmax.assign(max_slot, 0);
+        pmax_stamp.assign(max_slot, 0);
+        epoch = 0;
+    }
+    uint32_t seg_epoch = ++epoch;   /* slot -> intra item / "has items" in this segment */
     std::vector<uint32_t> seg_starts{0};
     size_t data_at = p.o_data;
     int64_t pts_acc = 0;
@@ -1271,12 +1283,14 @@ int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_ba
         p.frame_pts.push_back(static_cast<int64_t>(fr->points));
         p.frame_data_at.push_back(data_at);
         bool new_seg = false;
-        for (uint32_t s = 0; s < fr->slice_cnt; ++s)
-            if (!(fr->type[s] & PVVC_TYPE_PREDICT) && seen.count(slot_base[f] + fr->index[s])) new_seg = true;
+        for (uint32_t s = 0; s < fr->slice_cnt; ++s) {
+            const int pi = fr->index[s];
+            if (pi < 0 || pi >= slot_count[f]) return PVVC_ERR_INVALID_ARG;
+            if (!(fr->type[s] & PVVC_TYPE_PREDICT) && seen_stamp[slot_base[f] + pi] == seg_epoch) new_seg = true;
+        }
         if (new_seg && it != 0) {
             seg_starts.push_back(it);
-            intra_item.clear();
-            seen.clear();
+            seg_epoch = ++epoch;
         }
         int64_t inner = 0;
         for (uint32_t s = 0; s < fr->slice_cnt; ++s, ++it) {
@@ -1299,10 +1313,10 @@ int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_ba
             gcap_of[it] = geo_cap[f];
             if (x.type & PVVC_TYPE_PREDICT) {
                 if (intra_geo) p.state_bytes += static_cast<int64_t>(intra_geo[slot]) + 12ll * x.n;
-                auto ii = intra_item.find(slot);
-                if (ii != intra_item.end()) {
-                    p.geo_at[it] = p.geo_at[ii->second];
-                    x.geo_size = p.items[ii->second].geo_size;
+                if (intra_stamp[slot] == seg_epoch) {
+                    const uint32_t ii = intra_at[slot];
+                    p.geo_at[it] = p.geo_at[ii];
+                    x.geo_size = p.items[ii].geo_size;
                     x.wait = 1;
                 } else {
                     p.geo_at[it] = UINT64_MAX;
@@ -1310,9 +1324,10 @@ int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_ba
                 }
             } else {
                 if (intra_geo) intra_geo[slot] = x.geo_size;
-                intra_item[slot] = it;
+                intra_at[slot] = it;
+                intra_stamp[slot] = seg_epoch;
             }
-            seen[slot] = f;
+            seen_stamp[slot] = seg_epoch;
             inner += fr->size[s];
         }
         pts_acc += inner;
@@ -1326,19 +1341,26 @@ int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_ba
     seg_starts.push_back(n_items);
     for (size_t si = 0; si + 1 < seg_starts.size(); ++si) {
         const uint32_t lo = seg_starts[si], hi = seg_starts[si + 1];
+        const uint32_t pm_epoch = ++epoch;
+        for (uint32_t i = lo; i < hi; ++i) {
+            const int sl = p.items[i].patch;
+            if (pmax_stamp[sl] != pm_epoch) {
+                pmax_stamp[sl] = pm_epoch;
+                pmax[sl] = 0;
+            }
+            pmax[sl] = std::max(pmax[sl], p.items[i].n);
+        }
+        /* key: larger first, intra first on ties, then item order */
+        std::vector<uint64_t> keys(hi - lo);
+        for (uint32_t i = lo; i < hi; ++i) {
+            const ItemDev& A = p.items[i];
+            const bool pa = A.type & PVVC_TYPE_PREDICT;
+            const uint32_t ka = pa ? A.n : pmax[A.patch];
+            keys[i - lo] = (static_cast<uint64_t>(0xffffffffu - ka) << 32) | (static_cast<uint64_t>(pa) << 31) | i;
+        }
+        std::sort(keys.begin(), keys.end());
         std::vector<uint32_t> ids(hi - lo);
-        std::iota(ids.begin(), ids.end(), lo);
-        std::map<int, uint32_t> patch_max;
-        for (uint32_t i : ids) patch_max[p.items[i].patch] = std::max(patch_max[p.items[i].patch], p.items[i].n);
-        std::stable_sort(ids.begin(), ids.end(), [&](uint32_t a, uint32_t c) {
-            const ItemDev &A = p.items[a], &C = p.items[c];
-            const bool pa = A.type & PVVC_TYPE_PREDICT, pc = C.type & PVVC_TYPE_PREDICT;
-            const uint32_t ka = pa ? A.n : patch_max[A.patch];
-            const uint32_t kc = pc ? C.n : patch_max[C.patch];
-            if (ka != kc) return ka > kc;
-            if (pa != pc) return !pa;
-            return a < c;
-        });
+        for (uint32_t j = 0; j < hi - lo; ++j) ids[j] = static_cast<uint32_t>(keys[j] & 0x7fffffffu);
         uint64_t need = 0;
         for (uint32_t i : ids) {
             const ItemDev& x = p.items[i];

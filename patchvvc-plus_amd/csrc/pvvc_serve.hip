@@ -160,6 +160,7 @@ struct pvvc_server {
     uint32_t launch_id = 0;
     size_t rr = 0;
     int64_t frames_done = 0, points_done = 0, batches = 0;
+    double host_s = 0;
     std::vector<double> lat_ms;
     double first_arrival = -1, last_done = 0;
 };
@@ -366,6 +367,7 @@ void scheduler(pvvc_server* sv) {
             if (pts >= sv->cfg.max_batch_points) break;
         }
         if (frs.empty()) continue;
+        const double t_host0 = now_s();
         ln.busy = true;
         const pvvc_tuning tun = sv->cfg.tuning;
         lk.unlock();
@@ -426,6 +428,7 @@ void scheduler(pvvc_server* sv) {
         ln.launch_t = now_s();
         ln.has_work = true;
         sv->batches++;
+        sv->host_s += ln.launch_t - t_host0;
         ln.cv.notify_one();
     }
 }
@@ -507,9 +510,9 @@ int pvvc_server_create(const pvvc_server_config* cfg_in, pvvc_server** out) {
     pvvc_server_config cfg{};
     if (cfg_in) cfg = *cfg_in;
     if (cfg.lanes <= 0) cfg.lanes = 3;
-    if (cfg.max_slots <= 0) cfg.max_slots = 1 << 16;
+    if (cfg.max_slots <= 0) cfg.max_slots = 1 << 18;
     if (cfg.max_points_per_patch <= 0) cfg.max_points_per_patch = 4096;
-    if (cfg.out_pool_points <= 0) cfg.out_pool_points = int64_t{64} << 20;
+    if (cfg.out_pool_points <= 0) cfg.out_pool_points = int64_t{256} << 20;
     if (cfg.max_batch_points <= 0) cfg.max_batch_points = int64_t{16} << 20;
     if (cfg.tuning.waves_per_patch <= 0) cfg.tuning.waves_per_patch = 1;
     if (cfg.tuning.lds_bytes <= 0) cfg.tuning.lds_bytes = 8 * 1024;
@@ -530,7 +533,8 @@ int pvvc_server_create(const pvvc_server_config* cfg_in, pvvc_server** out) {
     for (Lane& ln : sv->lanes) {
         if (!good) break;
         good = ok(hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking)) &&
-               ok(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming | hipEventBlockingSync)) &&
+               ok(hipEventCreateWithFlags(&ln.done, sv->cfg.lanes_spin ? hipEventDisableTiming
+                                                                         : (hipEventDisableTiming | hipEventBlockingSync))) &&
                ok(hipMalloc(&ln.slab, need * grid)) && ok(hipMalloc(&ln.stage, 12ull * cfg.max_batch_points)) &&
                ok(hipMalloc(&ln.queue, 64));
         ln.slab_bytes = need;
@@ -691,7 +695,9 @@ int pvvc_server_get_stats(pvvc_server* sv, pvvc_server_stats* out) {
     for (auto& kv : sv->streams) {
         if (kv.second.state == ST_ACTIVE || kv.second.state == ST_CLOSING) out->streams_active++;
         if (kv.second.state == ST_PENDING) out->streams_waiting++;
+        out->backlog_frames += static_cast<int64_t>(kv.second.pending.size());
     }
+    out->host_ms_per_batch = sv->batches ? sv->host_s * 1e3 / static_cast<double>(sv->batches) : 0.0;
     std::vector<double> v = sv->lat_ms;
     if (!v.empty()) {
         std::sort(v.begin(), v.end());
@@ -714,6 +720,7 @@ int pvvc_server_reset_stats(pvvc_server* sv) {
     if (!sv) return PVVC_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(sv->mu);
     sv->frames_done = sv->points_done = sv->batches = 0;
+    sv->host_s = 0;
     sv->lat_ms.clear();
     sv->first_arrival = -1;
     sv->last_done = 0;

@@ -392,10 +392,12 @@ class Server:
     completion has been polled."""
 
     def __init__(self, hbm_budget: int = 0, lanes: int = 0, max_slots: int = 0, max_points_per_patch: int = 0,
-                 out_pool_points: int = 0, max_batch_points: int = 0, tuning: dict | None = None):
+                 out_pool_points: int = 0, max_batch_points: int = 0, tuning: dict | None = None,
+                 lanes_spin: bool = True):
         from ._native import ServerConfig
         t = Tuning(**(tuning or {}))
-        cfg = ServerConfig(hbm_budget, lanes, max_slots, max_points_per_patch, out_pool_points, max_batch_points, t)
+        cfg = ServerConfig(hbm_budget, lanes, max_slots, max_points_per_patch, out_pool_points, max_batch_points, t,
+                           int(lanes_spin))
         h = C.c_void_p()
         _check(hip_lib().pvvc_server_create(C.byref(cfg), C.byref(h)), "pvvc_server_create")
         self.handle = h

@@ -39,7 +39,7 @@ class Tuning(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("lanes", C.c_int), ("max_slots", C.c_int),
                 ("max_points_per_patch", C.c_int), ("out_pool_points", C.c_int64), ("max_batch_points", C.c_int64),
-                ("tuning", Tuning)]
+                ("tuning", Tuning), ("lanes_spin", C.c_int)]
 
 
 class Completion(C.Structure):
@@ -52,7 +52,8 @@ class ServerStats(C.Structure):
     _fields_ = [("frames_done", C.c_int64), ("points_done", C.c_int64), ("batches", C.c_int64),
                 ("hbm_budget", C.c_uint64), ("hbm_used", C.c_uint64), ("streams_active", C.c_int),
                 ("streams_waiting", C.c_int), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
-                ("max_ms", C.c_double), ("mpts_per_s", C.c_double)]
+                ("max_ms", C.c_double), ("mpts_per_s", C.c_double), ("host_ms_per_batch", C.c_double),
+                ("backlog_frames", C.c_int64)]
 
 
 class SynthConfig(C.Structure):

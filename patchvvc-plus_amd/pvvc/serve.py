@@ -92,6 +92,9 @@ def run(server: Server, specs: list[StreamSpec], fps: float = 30.0, duration: fl
     th.join(timeout=max(30.0, 4 * duration))
     stop.set()
     st = server.stats()
+    if st["streams_waiting"]:
+        # streams that never fit the budget / slot table / output pool stay waiting
+        st["note"] = f"{st['streams_waiting']} streams were never admitted"
     for sid in ids:
         server.close_stream(sid)
     q = lambda a, p: float(np.percentile(np.asarray(a) * 1e3, p)) if len(a) else 0.0  # noqa: E731
