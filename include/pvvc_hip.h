@@ -113,6 +113,11 @@ int pvvc_load_frame(const char* path, pvvc_frame** out);
 /* Parses a .frame image already in host memory (same codes, 1-4 unused). */
 int pvvc_parse_frame(const uint8_t* bytes, size_t nbytes, pvvc_frame** out);
 void pvvc_frame_free(pvvc_frame* frame);
+/* Host threads that unwrap a frame's zstd blobs in parallel (default:
+ * min(16, hardware threads); 1 unwraps on the calling thread only).  The
+ * reference loader unwraps slice by slice on one thread
+ * (src/cuda/common/slice_io.cu:225-270). */
+int pvvc_set_host_threads(int n);
 int pvvc_frame_timestamp(const pvvc_frame* frame);
 uint32_t pvvc_frame_slice_cnt(const pvvc_frame* frame);
 uint64_t pvvc_frame_points(const pvvc_frame* frame);

@@ -19,12 +19,118 @@
 
 #include <zstd.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cerrno>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <new>
 #include <regex>
+#include <thread>
 
 namespace {
+
+/* Host worker pool for the zstd unwrap: parallel_for(n, fn) runs fn(i) for
+ * i in [0, n) on the workers and the calling thread; several callers may
+ * share the pool at once. */
+class HostPool {
+  public:
+    void set_threads(int n) {
+        std::lock_guard<std::mutex> lk(cfg_mu_);
+        want_ = std::max(1, std::min(n, 64));
+    }
+    int threads() {
+        std::lock_guard<std::mutex> lk(cfg_mu_);
+        return want_;
+    }
+    void parallel_for(size_t n, const std::function<void(size_t)>& fn) {
+        ensure_started();
+        if (n == 0) return;
+        if (n == 1 || threads() <= 1) {
+            for (size_t i = 0; i < n; ++i) fn(i);
+            return;
+        }
+        Job job;
+        job.fn = &fn;
+        job.n = n;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.push_back(&job);
+        }
+        cv_.notify_all();
+        run(job);
+        /* the job lives on this stack: wait until every index ran and no
+         * worker still holds it */
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return job.done.load() == n && job.holders == 0; });
+        auto it = std::find(q_.begin(), q_.end(), &job);
+        if (it != q_.end()) q_.erase(it);
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+
+  private:
+    struct Job {
+        const std::function<void(size_t)>* fn = nullptr;
+        size_t n = 0;
+        std::atomic<size_t> next{0}, done{0};
+        int holders = 0;   /* workers between taking the job and leaving run(); under mu_ */
+    };
+    void run(Job& j) {
+        size_t i;
+        while ((i = j.next.fetch_add(1)) < j.n) {
+            (*j.fn)(i);
+            if (j.done.fetch_add(1) + 1 == j.n) {
+                std::lock_guard<std::mutex> lk(mu_);
+                done_cv_.notify_all();
+            }
+        }
+    }
+    void worker() {
+        for (;;) {
+            Job* j = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] {
+                    while (!q_.empty() && q_.front()->next.load() >= q_.front()->n) q_.pop_front();
+                    return stop_ || !q_.empty();
+                });
+                if (stop_) return;
+                j = q_.front();
+                j->holders++;
+            }
+            run(*j);
+            std::lock_guard<std::mutex> lk(mu_);
+            j->holders--;
+            done_cv_.notify_all();
+        }
+    }
+    void ensure_started() {
+        std::lock_guard<std::mutex> lk(cfg_mu_);
+        while (static_cast<int>(th_.size()) < want_ - 1) th_.emplace_back([this] { worker(); });
+    }
+    std::mutex cfg_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    std::deque<Job*> q_;
+    std::vector<std::thread> th_;
+    int want_ = std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
+    bool stop_ = false;
+};
+
+HostPool& host_pool() {
+    static HostPool pool;
+    return pool;
+}
 
 /* Bounds-checked little-endian reader over a memory image. */
 struct Cursor {
@@ -46,15 +152,22 @@ struct Cursor {
 
 size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-/* ZstdDecoder::Decode semantics (src/common/entropy_codec.cc:341-360). */
-bool zstd_unwrap(const uint8_t* src, size_t n, std::vector<uint8_t>& out) {
-    unsigned long long sz = ZSTD_getFrameContentSize(src, n);
-    if (sz == 0 || sz == ZSTD_CONTENTSIZE_UNKNOWN || sz == ZSTD_CONTENTSIZE_ERROR) return false;
-    out.resize(sz);
-    size_t r = ZSTD_decompress(out.data(), sz, src, n);
-    if (ZSTD_isError(r)) return false;
-    out.resize(r);
+/* ZstdDecoder::Decode semantics (src/common/entropy_codec.cc:341-360): the
+ * frame header states the size; 0 or unknown is an error. */
+bool zstd_size(const uint8_t* src, size_t n, size_t& sz) {
+    unsigned long long v = ZSTD_getFrameContentSize(src, n);
+    if (v == 0 || v == ZSTD_CONTENTSIZE_UNKNOWN || v == ZSTD_CONTENTSIZE_ERROR) return false;
+    if (v > (1ull << 32)) return false;
+    sz = static_cast<size_t>(v);
     return true;
+}
+
+/* Decompresses into dst[0, cap); returns the size or -1. */
+int64_t zstd_into(const uint8_t* src, size_t n, uint8_t* dst, size_t cap) {
+    thread_local ZSTD_DCtx* dctx = ZSTD_createDCtx();
+    if (!dctx) return -1;
+    size_t r = ZSTD_decompressDCtx(dctx, dst, cap, src, n);
+    return ZSTD_isError(r) ? -1 : static_cast<int64_t>(r);
 }
 
 }  // namespace
@@ -89,36 +202,62 @@ int pvvc_frame_parse_image(const uint8_t* bytes, size_t nbytes, pvvc_frame& f) {
         if (!raw_col[i]) continue;
         if (!(csrc[i] = c.span(raw_col[i]))) return 5;
     }
-    /* Unwrap into one arena; sizes become the unwrapped sizes (slice_io.cu:239,264). */
-    std::vector<std::vector<uint8_t>> tmp_g(S), tmp_c(S);
+    /* Unwrap into one arena; sizes become the unwrapped sizes (slice_io.cu:239,264).
+     * The zstd frame headers give every blob's size, so the arena is laid out
+     * first and each blob is decompressed (in parallel) straight into place. */
+    std::vector<uint8_t> gz(S), cz(S);
     size_t total = 0;
     for (uint32_t i = 0; i < S; ++i) {
-        if (gsrc[i] && (f.type[i] & PVVC_TYPE_GEO_ZSTD)) {
-            if (!zstd_unwrap(gsrc[i], raw_geo[i], tmp_g[i])) return 6;
-            f.geo_size[i] = static_cast<uint32_t>(tmp_g[i].size());
-        } else {
-            f.geo_size[i] = raw_geo[i];
-        }
-        if (csrc[i] && (f.type[i] & PVVC_TYPE_COLOR_ZSTD)) {
-            if (!zstd_unwrap(csrc[i], raw_col[i], tmp_c[i])) return 6;
-            f.col_size[i] = static_cast<uint32_t>(tmp_c[i].size());
-        } else {
-            f.col_size[i] = raw_col[i];
-        }
-        total += align16(f.geo_size[i]) + align16(f.col_size[i]);
+        size_t gs = raw_geo[i], cs = raw_col[i];
+        gz[i] = gsrc[i] && (f.type[i] & PVVC_TYPE_GEO_ZSTD);
+        cz[i] = csrc[i] && (f.type[i] & PVVC_TYPE_COLOR_ZSTD);
+        if (gz[i] && !zstd_size(gsrc[i], raw_geo[i], gs)) return 6;
+        if (cz[i] && !zstd_size(csrc[i], raw_col[i], cs)) return 6;
+        f.geo_off[i] = total;
+        f.geo_size[i] = static_cast<uint32_t>(gs);
+        total += align16(gs);
+        f.col_off[i] = total;
+        f.col_size[i] = static_cast<uint32_t>(cs);
+        total += align16(cs);
     }
-    f.data.assign(total + 16, 0);
-    size_t at = 0;
+    /* a zstd header may claim any size: refuse arenas far beyond the image */
+    if (total > (size_t{1} << 30) && total / 1024 > nbytes) return 6;
+    try {
+        f.data.assign(total + 16, 0);
+    } catch (const std::bad_alloc&) {
+        return 6;
+    }
+    std::atomic<int> bad{0};
+    auto blob = [&](size_t i) {
+        uint8_t* base = f.data.data();
+        if (gsrc[i]) {
+            if (gz[i]) {
+                const int64_t r = zstd_into(gsrc[i], raw_geo[i], base + f.geo_off[i], f.geo_size[i]);
+                if (r < 0) bad = 1;
+                else f.geo_size[i] = static_cast<uint32_t>(r);
+            } else {
+                std::memcpy(base + f.geo_off[i], gsrc[i], f.geo_size[i]);
+            }
+        }
+        if (csrc[i]) {
+            if (cz[i]) {
+                const int64_t r = zstd_into(csrc[i], raw_col[i], base + f.col_off[i], f.col_size[i]);
+                if (r < 0) bad = 1;
+                else f.col_size[i] = static_cast<uint32_t>(r);
+            } else {
+                std::memcpy(base + f.col_off[i], csrc[i], f.col_size[i]);
+            }
+        }
+    };
+    /* tasks of 64 slices: a blob is ~1 KB, too little to hand out alone */
+    constexpr size_t kChunk = 64;
+    host_pool().parallel_for((S + kChunk - 1) / kChunk, [&](size_t t) {
+        const size_t hi = std::min<size_t>(S, (t + 1) * kChunk);
+        for (size_t i = t * kChunk; i < hi; ++i) blob(i);
+    });
+    if (bad.load()) return 6;
     f.points = 0;
-    for (uint32_t i = 0; i < S; ++i) {
-        f.geo_off[i] = at;
-        if (f.geo_size[i]) std::memcpy(&f.data[at], tmp_g[i].empty() ? gsrc[i] : tmp_g[i].data(), f.geo_size[i]);
-        at += align16(f.geo_size[i]);
-        f.col_off[i] = at;
-        if (f.col_size[i]) std::memcpy(&f.data[at], tmp_c[i].empty() ? csrc[i] : tmp_c[i].data(), f.col_size[i]);
-        at += align16(f.col_size[i]);
-        f.points += f.size[i];
-    }
+    for (uint32_t i = 0; i < S; ++i) f.points += f.size[i];
     return 0;
 }
 
@@ -160,6 +299,12 @@ int pvvc_load_frame(const char* path, pvvc_frame** out) {
 }
 
 void pvvc_frame_free(pvvc_frame* f) { delete f; }
+
+int pvvc_set_host_threads(int n) {
+    if (n <= 0) return PVVC_ERR_INVALID_ARG;
+    host_pool().set_threads(n);
+    return PVVC_OK;
+}
 int pvvc_frame_timestamp(const pvvc_frame* f) { return f ? f->timestamp : -1; }
 uint32_t pvvc_frame_slice_cnt(const pvvc_frame* f) { return f ? f->slice_cnt : 0; }
 uint64_t pvvc_frame_points(const pvvc_frame* f) { return f ? f->points : 0; }

@@ -175,6 +175,11 @@ def last_kernel_ms() -> float:
     return float(hip_lib().pvvc_last_kernel_ms())
 
 
+def set_host_threads(n: int) -> None:
+    """Host threads used to unwrap a frame's zstd blobs."""
+    _check(hip_lib().pvvc_set_host_threads(int(n)), "pvvc_set_host_threads")
+
+
 def last_launch_times() -> list[float]:
     L = hip_lib()
     n = L.pvvc_last_launch_times(None, 0)

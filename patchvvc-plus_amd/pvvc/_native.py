@@ -85,6 +85,7 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_load_frame", C.c_int, C.c_char_p, C.POINTER(c_vp))
     _sig(L, "pvvc_parse_frame", C.c_int, c_vp, C.c_size_t, C.POINTER(c_vp))
     _sig(L, "pvvc_frame_free", None, c_vp)
+    _sig(L, "pvvc_set_host_threads", C.c_int, C.c_int)
     _sig(L, "pvvc_frame_timestamp", C.c_int, c_vp)
     _sig(L, "pvvc_frame_slice_cnt", C.c_uint32, c_vp)
     _sig(L, "pvvc_frame_points", C.c_uint64, c_vp)
