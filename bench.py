@@ -293,6 +293,52 @@ def main() -> int:
     log(f"per-frame drop-in: {np.mean(frame_kernel_ms):.3f} ms kernel, p50 {pct(lat, 50):.3f} p99 {pct(lat, 99):.3f} ms,"
         f" {pf_fps:.1f} fps")
 
+    # ---------------- end to end per frame: container + H2D + decode ----------------
+    # The reference harness times its per-slice copies plus the decode
+    # (test_decode_only.cc:93-158, t0..t3); here one packed pinned copy per
+    # frame.  "with_parse" also parses the .frame image (zstd unwrap) on the
+    # host; "pipelined" parses frame k+1 while the GPU decodes frame k.
+    e2e = None
+    if run_p:
+        cap = max(pvvc.Stager.bytes_for(f) for f in frames)
+        stagers = [pvvc.Stager(cap), pvvc.Stager(cap)]
+        state_e = pvvc.State(n_patches, max_pts)
+        n_e = args.steps * gop
+        seq = [i % len(images) for i in range(n_e)]
+        for k in range(min(gop, len(images))):  # warm
+            pvvc.launch_dropin(stagers[0].stage(frames[k]), state_e, out_pts)
+            pvvc.synchronize()
+        lat_ref, lat_all = [], []
+        for i, k in enumerate(seq):
+            t0 = time.perf_counter()
+            f = pvvc.Frame.parse(images[k])
+            t1 = time.perf_counter()
+            pvvc.launch_dropin(stagers[0].stage(f), state_e, out_pts)
+            pvvc.synchronize()
+            t2 = time.perf_counter()
+            lat_ref.append((t2 - t1) * 1e3)
+            lat_all.append((t2 - t0) * 1e3)
+        t0 = time.perf_counter()
+        nxt = pvvc.Frame.parse(images[seq[0]])
+        for i in range(n_e):
+            f = nxt
+            pvvc.launch_dropin(stagers[i & 1].stage(f), state_e, out_pts)
+            if i + 1 < n_e:
+                nxt = pvvc.Frame.parse(images[seq[i + 1]])
+            pvvc.synchronize()
+        pipe_s = time.perf_counter() - t0
+        e2e = {"frames": n_e,
+               "h2d_decode": {"frames_per_s": round(1e3 / float(np.mean(lat_ref)), 1),
+                              "p50_ms": round(pct(lat_ref, 50), 4), "p99_ms": round(pct(lat_ref, 99), 4)},
+               "with_parse": {"frames_per_s": round(1e3 / float(np.mean(lat_all)), 1),
+                              "p50_ms": round(pct(lat_all, 50), 4), "p99_ms": round(pct(lat_all, 99), 4)},
+               "pipelined_frames_per_s": round(n_e / pipe_s, 1),
+               "pipelined_mpts_per_s": round(sum(pts_per_frame[k] for k in seq) / pipe_s / 1e6, 2),
+               "staged_bytes_per_frame": int(np.mean([pvvc.Stager.bytes_for(f) for f in frames]))}
+        for st_ in stagers:
+            st_.close()
+        log("end to end", e2e)
+
     # ---------------- serving loop: many concurrent streams ----------------
     serving = None
     if not args.no_serving and args.only == "all":
@@ -390,6 +436,7 @@ def main() -> int:
                      "traffic_detail": traffic},
         "state_bytes_per_step": int(np.mean(state_bytes)),
         "rlgr_symbols_per_s_per_cu": round(sym_per_s_cu, 1),
+        "end_to_end_per_frame": e2e,
         "accuracy": accuracy,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -148,6 +148,18 @@ typedef struct pvvc_dropin_frame {
 } pvvc_dropin_frame;
 int pvvc_dropin_stage(const pvvc_frame* frame, pvvc_dropin_frame* dev);
 void pvvc_dropin_release(pvvc_dropin_frame* dev);
+/* Bytes of the packed image pvvc_dropin_stage_to needs for this frame. */
+size_t pvvc_dropin_stage_bytes(const pvvc_frame* frame);
+/* Stages into caller buffers, allocating nothing: packs the image into
+ * `host` (pinned memory from pvvc_host_alloc makes the copy asynchronous)
+ * and copies it to `dev` on the library stream.  `host` may be reused once
+ * the stream has passed the copy.  PVVC_ERR_CAPACITY if either buffer is
+ * smaller than pvvc_dropin_stage_bytes(frame). */
+int pvvc_dropin_stage_to(const pvvc_frame* frame, void* dev, size_t dev_bytes, void* host, size_t host_bytes,
+                         pvvc_dropin_frame* out);
+/* Pinned host memory (hipHostMalloc). */
+void* pvvc_host_alloc(size_t bytes);
+void pvvc_host_free(void* ptr);
 
 /* ------------------------------------------------------------------------ */
 /* Native decoder: batched persistent decode                                 */

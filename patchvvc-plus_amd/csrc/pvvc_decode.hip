@@ -1697,60 +1697,115 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
 }
 
 /* ---- drop-in staging (test_decode_only.cc:97-123 in one copy) ---- */
-int pvvc_dropin_stage(const pvvc_frame* f, pvvc_dropin_frame* d) {
-    if (!f || !d) return PVVC_ERR_INVALID_ARG;
-    std::memset(d, 0, sizeof(*d));
+}  // extern "C"
+
+namespace {
+
+/* Packed image of one drop-in call: [ptr arrays][u32/int arrays][u8 arrays][mv][blobs]. */
+struct StageLayout {
+    size_t o_mvp, o_geop, o_colp, o_off, o_idx, o_size, o_gsz, o_csz, o_type, o_qp, o_mv, o_data, total;
+    explicit StageLayout(const pvvc_frame* f) {
+        const size_t S = f->slice_cnt;
+        auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+        o_mvp = 0;
+        o_geop = al(o_mvp + 8 * S);
+        o_colp = al(o_geop + 8 * S);
+        o_off = al(o_colp + 8 * S);
+        o_idx = al(o_off + 4 * S);
+        o_size = al(o_idx + 4 * S);
+        o_gsz = al(o_size + 4 * S);
+        o_csz = al(o_gsz + 4 * S);
+        o_type = al(o_csz + 4 * S);
+        o_qp = al(o_type + S);
+        o_mv = al(o_qp + S);
+        o_data = al(o_mv + 64 * S);
+        total = o_data + f->data.size();
+    }
+};
+
+/* Fills the host image for a device copy at `dev` and the call's device arrays. */
+void stage_fill(const pvvc_frame* f, const StageLayout& L, uint8_t* dev, uint8_t* h, pvvc_dropin_frame* d) {
     const uint32_t S = f->slice_cnt;
-    /* layout: [ptr arrays][u32/int arrays][u8 arrays][mv][blobs] */
-    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-    size_t o_mvp = 0, o_geop = al(o_mvp + 8 * S), o_colp = al(o_geop + 8 * S);
-    size_t o_off = al(o_colp + 8 * S), o_idx = al(o_off + 4 * S), o_size = al(o_idx + 4 * S);
-    size_t o_gsz = al(o_size + 4 * S), o_csz = al(o_gsz + 4 * S), o_type = al(o_csz + 4 * S);
-    size_t o_qp = al(o_type + S), o_mv = al(o_qp + S), o_data = al(o_mv + 64 * S);
-    size_t total = o_data + f->data.size();
-    uint8_t* dev = static_cast<uint8_t*>(pvvc_dev_alloc(total));
-    if (!dev) return PVVC_ERR_HIP;
-    std::vector<uint8_t> h(total, 0);
-    std::vector<int> off(S);
     int64_t acc = 0;
+    int* off = reinterpret_cast<int*>(h + L.o_off);
     for (uint32_t i = 0; i < S; ++i) {
         off[i] = static_cast<int>(acc);
         acc += f->size[i];
-        uint64_t mvp = reinterpret_cast<uint64_t>(dev + o_mv + 64 * i);
-        uint64_t gp = f->geo_size[i] ? reinterpret_cast<uint64_t>(dev + o_data + f->geo_off[i]) : 0;
-        uint64_t cp = f->col_size[i] ? reinterpret_cast<uint64_t>(dev + o_data + f->col_off[i]) : 0;
-        std::memcpy(&h[o_mvp + 8 * i], &mvp, 8);
-        std::memcpy(&h[o_geop + 8 * i], &gp, 8);
-        std::memcpy(&h[o_colp + 8 * i], &cp, 8);
+        const uint64_t mvp = reinterpret_cast<uint64_t>(dev + L.o_mv + 64 * i);
+        const uint64_t gp = f->geo_size[i] ? reinterpret_cast<uint64_t>(dev + L.o_data + f->geo_off[i]) : 0;
+        const uint64_t cp = f->col_size[i] ? reinterpret_cast<uint64_t>(dev + L.o_data + f->col_off[i]) : 0;
+        std::memcpy(h + L.o_mvp + 8 * i, &mvp, 8);
+        std::memcpy(h + L.o_geop + 8 * i, &gp, 8);
+        std::memcpy(h + L.o_colp + 8 * i, &cp, 8);
     }
-    std::memcpy(&h[o_off], off.data(), 4 * S);
-    std::memcpy(&h[o_idx], f->index.data(), 4 * S);
-    std::memcpy(&h[o_size], f->size.data(), 4 * S);
-    std::memcpy(&h[o_gsz], f->geo_size.data(), 4 * S);
-    std::memcpy(&h[o_csz], f->col_size.data(), 4 * S);
-    std::memcpy(&h[o_type], f->type.data(), S);
-    std::memcpy(&h[o_qp], f->qp.data(), S);
-    std::memcpy(&h[o_mv], f->mv.data(), 64 * S);
-    std::memcpy(&h[o_data], f->data.data(), f->data.size());
-    if (pvvc_memcpy_h2d(dev, h.data(), total) != PVVC_OK) {
-        pvvc_dev_free(dev);
-        return PVVC_ERR_HIP;
-    }
-    d->mv = reinterpret_cast<float**>(dev + o_mvp);
-    d->geometry = reinterpret_cast<uint8_t**>(dev + o_geop);
-    d->color = reinterpret_cast<uint8_t**>(dev + o_colp);
-    d->inner_offset = reinterpret_cast<int*>(dev + o_off);
-    d->index = reinterpret_cast<int*>(dev + o_idx);
-    d->size = reinterpret_cast<uint32_t*>(dev + o_size);
-    d->geometry_size = reinterpret_cast<uint32_t*>(dev + o_gsz);
-    d->color_size = reinterpret_cast<uint32_t*>(dev + o_csz);
-    d->type = dev + o_type;
-    d->qp = dev + o_qp;
+    std::memcpy(h + L.o_idx, f->index.data(), 4ull * S);
+    std::memcpy(h + L.o_size, f->size.data(), 4ull * S);
+    std::memcpy(h + L.o_gsz, f->geo_size.data(), 4ull * S);
+    std::memcpy(h + L.o_csz, f->col_size.data(), 4ull * S);
+    std::memcpy(h + L.o_type, f->type.data(), S);
+    std::memcpy(h + L.o_qp, f->qp.data(), S);
+    std::memcpy(h + L.o_mv, f->mv.data(), 64ull * S);
+    std::memcpy(h + L.o_data, f->data.data(), f->data.size());
+    d->mv = reinterpret_cast<float**>(dev + L.o_mvp);
+    d->geometry = reinterpret_cast<uint8_t**>(dev + L.o_geop);
+    d->color = reinterpret_cast<uint8_t**>(dev + L.o_colp);
+    d->inner_offset = reinterpret_cast<int*>(dev + L.o_off);
+    d->index = reinterpret_cast<int*>(dev + L.o_idx);
+    d->size = reinterpret_cast<uint32_t*>(dev + L.o_size);
+    d->geometry_size = reinterpret_cast<uint32_t*>(dev + L.o_gsz);
+    d->color_size = reinterpret_cast<uint32_t*>(dev + L.o_csz);
+    d->type = dev + L.o_type;
+    d->qp = dev + L.o_qp;
     d->slice_cnt = static_cast<int>(S);
     d->point_cnt = acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pvvc_dropin_stage(const pvvc_frame* f, pvvc_dropin_frame* d) {
+    if (!f || !d) return PVVC_ERR_INVALID_ARG;
+    std::memset(d, 0, sizeof(*d));
+    const StageLayout L(f);
+    uint8_t* dev = static_cast<uint8_t*>(pvvc_dev_alloc(L.total));
+    if (!dev) return PVVC_ERR_HIP;
+    std::vector<uint8_t> h(L.total, 0);
+    stage_fill(f, L, dev, h.data(), d);
+    if (pvvc_memcpy_h2d(dev, h.data(), L.total) != PVVC_OK) {
+        pvvc_dev_free(dev);
+        std::memset(d, 0, sizeof(*d));
+        return PVVC_ERR_HIP;
+    }
     d->blob = dev;
-    d->blob_bytes = total;
+    d->blob_bytes = L.total;
     return PVVC_OK;
+}
+
+size_t pvvc_dropin_stage_bytes(const pvvc_frame* f) { return f ? StageLayout(f).total : 0; }
+
+int pvvc_dropin_stage_to(const pvvc_frame* f, void* dev, size_t dev_bytes, void* host, size_t host_bytes,
+                         pvvc_dropin_frame* d) {
+    if (!f || !d || !dev || !host) return PVVC_ERR_INVALID_ARG;
+    std::memset(d, 0, sizeof(*d));
+    const StageLayout L(f);
+    if (L.total > dev_bytes || L.total > host_bytes) return PVVC_ERR_CAPACITY;
+    stage_fill(f, L, static_cast<uint8_t*>(dev), static_cast<uint8_t*>(host), d);
+    if (!hip_ok(hipMemcpyAsync(dev, host, L.total, hipMemcpyHostToDevice, g_stream))) {
+        std::memset(d, 0, sizeof(*d));
+        return PVVC_ERR_HIP;
+    }
+    d->blob = nullptr; /* caller-owned */
+    d->blob_bytes = L.total;
+    return PVVC_OK;
+}
+
+void* pvvc_host_alloc(size_t bytes) {
+    void* p = nullptr;
+    return hip_ok(hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault)) ? p : nullptr;
+}
+void pvvc_host_free(void* p) {
+    if (p) hipHostFree(p);
 }
 
 void pvvc_dropin_release(pvvc_dropin_frame* d) {

@@ -243,6 +243,45 @@ class StagedFrame:
             pass
 
 
+class Stager:
+    """Reusable drop-in staging buffers: one device buffer and one pinned
+    host buffer (pvvc_dropin_stage_to); stage() allocates nothing."""
+
+    class Staged:
+        def __init__(self, d: DropinFrame):
+            self.d = d
+
+    def __init__(self, capacity: int):
+        self.capacity = int(capacity)
+        self.dev = DeviceBuffer(self.capacity)
+        self.host = hip_lib().pvvc_host_alloc(self.capacity)
+        if not self.host:
+            raise PvvcError(9, "pvvc_host_alloc")
+
+    @staticmethod
+    def bytes_for(frame: Frame) -> int:
+        return int(hip_lib().pvvc_dropin_stage_bytes(frame.handle))
+
+    def stage(self, frame: Frame) -> "Stager.Staged":
+        d = DropinFrame()
+        _check(hip_lib().pvvc_dropin_stage_to(frame.handle, C.c_void_p(self.dev.ptr), self.capacity,
+                                              C.c_void_p(self.host), self.capacity, C.byref(d)),
+               "pvvc_dropin_stage_to")
+        return Stager.Staged(d)
+
+    def close(self) -> None:
+        if self.host:
+            hip_lib().pvvc_host_free(C.c_void_p(self.host))
+            self.host = None
+        self.dev.free()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def launch_dropin(staged: StagedFrame, state: State, out: DeviceBuffer, timestamp: int = 0) -> None:
     """launch_cudaProcess with the staged arrays (asynchronous)."""
     d = staged.d

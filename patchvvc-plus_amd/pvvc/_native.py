@@ -103,6 +103,10 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_set_dropin_tuning", C.c_int, C.c_int, C.c_int)
     _sig(L, "pvvc_dropin_stage", C.c_int, c_vp, C.POINTER(DropinFrame))
     _sig(L, "pvvc_dropin_release", None, C.POINTER(DropinFrame))
+    _sig(L, "pvvc_dropin_stage_bytes", C.c_size_t, c_vp)
+    _sig(L, "pvvc_dropin_stage_to", C.c_int, c_vp, c_vp, C.c_size_t, c_vp, C.c_size_t, C.POINTER(DropinFrame))
+    _sig(L, "pvvc_host_alloc", c_vp, C.c_size_t)
+    _sig(L, "pvvc_host_free", None, c_vp)
     _sig(L, "pvvc_decoder_create", C.c_int, C.POINTER(c_vp), C.c_int, C.c_int)
     _sig(L, "pvvc_decoder_destroy", C.c_int, c_vp)
     _sig(L, "pvvc_decoder_set_tuning", C.c_int, c_vp, C.POINTER(Tuning))

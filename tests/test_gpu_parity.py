@@ -110,3 +110,25 @@ def test_autotune_keeps_output(pvvc, c0_files, c0_oracle):
         o, k = b.frame_offset[i], b.frame_points[i]
         np.testing.assert_array_equal(xyz[o:o + k], ref["xyz"])
         np.testing.assert_array_equal(rgb[o:o + k], ref["rgb"])
+
+
+def test_dropin_stage_to_matches_stage(pvvc, c0_files, c0_oracle):
+    """Staging into reusable caller buffers (pinned host + device, no
+    allocation per frame) decodes exactly like pvvc_dropin_stage."""
+    frames = [pvvc.Frame.load(str(p)) for p in c0_files]
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames)
+    cap = max(pvvc.Stager.bytes_for(f) for f in frames)
+    stager = pvvc.Stager(cap)
+    st = pvvc.State(n_patches, max_pts)
+    out = pvvc.DeviceBuffer(24 * max(f.points for f in frames))
+    for f, ref in zip(frames, c0_oracle):
+        s = stager.stage(f)
+        pvvc.launch_dropin(s, st, out)
+        pvvc.synchronize()
+        pts = out.read(pvvc.POINTS_DTYPE, f.points)
+        np.testing.assert_array_equal(np.stack([pts["x"], pts["y"], pts["z"]], 1), ref["xyz"])
+        np.testing.assert_array_equal(np.stack([pts["r"], pts["g"], pts["b"]], 1), ref["rgbf"] / np.float32(255))
+    with pytest.raises(pvvc.PvvcError):
+        pvvc.Stager(64).stage(frames[0])  # too small: capacity error, nothing copied
+    stager.close()
