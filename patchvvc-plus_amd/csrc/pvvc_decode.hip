@@ -145,6 +145,11 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  *    (Unsign2Sign is applied when the chunk is stored);
  *  - the window takes 32 more bits when fewer than 33 remain; the borrow of
  *    nbx -= length says so.
+ * The k = 0 and run cases are two loops (labels 10 and 30), each unrolled
+ * twice and handing over to the other when k_P crosses 3, so that the common
+ * path falls through: a taken scalar branch costs ~30 cycles on one
+ * wavefront, an instruction ~4.  Refills (11/13, 33/43) and the 1-bit run
+ * prefixes (31/41) are out of line.
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window, so every check precedes any state change.  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
@@ -158,29 +163,28 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
                                                uint32_t vrev) {
     int32_t vt;
     asm volatile(
-        /* ================= symbol start ================= */
-        "1:\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc1 30f\n\t"
-        /* ---- k = 0: GR(k_R) ---- */
+        "10:\n\t"
+        /* ---- k = 0 symbol ---- */
         "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"        /* ones; -1 if the window is all ones */
-        "s_add_u32 s86, s84, 1\n\t"                /* carry iff ones == -1 */
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
         "s_cbranch_scc1 9f\n\t"
-        "s_lshr_b32 s85, %[kRP], 2\n\t"            /* k_R */
-        "s_add_u32 s87, s86, s85\n\t"              /* code length */
+        "s_lshr_b32 s85, %[kRP], 2\n\t"
+        "s_add_u32 s87, s86, s85\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"                    /* escape / long code: C++ */
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"     /* drop the ones: [0][rem]... */
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"   /* rem (k_R bits) */
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
         "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"             /* ones << k_R; SCC = ones != 0 */
-        "s_cselect_b32 s90, -1, -2\n\t"            /* k_RP step: ones ? ones - 1 : -2 */
-        "s_or_b32 s89, s89, s82\n\t"               /* u; SCC = u != 0 */
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_cselect_b32 s90, -1, -2\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
         "s_cselect_b32 s88, -1, 3\n\t"
         "s_add_i32 %[kP], %[kP], s88\n\t"
-        "s_max_i32 %[kP], %[kP], 0\n\t"            /* k_P: u ? max(k_P - 1, 0) : k_P + 3 */
+        "s_max_i32 %[kP], %[kP], 0\n\t"
         "s_add_i32 s88, %[kRP], s84\n\t"
         "s_add_i32 s88, s88, s90\n\t"
         "s_max_i32 s88, s88, 0\n\t"
@@ -188,65 +192,109 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "v_mov_b32 %[vt], s89\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
         "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"        /* borrow iff nb < 33 after the code */
-        "s_cbranch_scc1 20f\n\t"
-        "10:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"          /* carry iff the chunk is complete */
-        "s_cbranch_scc0 1b\n\t"
-        "s_branch 9f\n\t"
-        /* refill (nb <= 32): 32 bits from dword wpos of the chunk */
-        "20:\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 11f\n\t"
+        "12:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc1 30f\n\t"
+        /* ---- k = 0 symbol ---- */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshr_b32 s85, %[kRP], 2\n\t"
+        "s_add_u32 s87, s86, s85\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s87\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_cselect_b32 s90, -1, -2\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
+        "s_cselect_b32 s88, -1, 3\n\t"
+        "s_add_i32 %[kP], %[kP], s88\n\t"
+        "s_max_i32 %[kP], %[kP], 0\n\t"
+        "s_add_i32 s88, %[kRP], s84\n\t"
+        "s_add_i32 s88, s88, s90\n\t"
+        "s_max_i32 s88, s88, 0\n\t"
+        "s_min_i32 %[kRP], s88, 128\n\t"
+        "v_mov_b32 %[vt], s89\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 13f\n\t"
+        "14:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc1 30f\n\t"
+        "s_branch 10b\n\t"
+        "11:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 21f\n\t"                   /* next chunk needed: C++ */
+        "s_cbranch_scc1 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"            /* 32 - nb */
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_nop 1\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
-        "s_branch 10b\n\t"
-        "21:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
-        "s_branch 9f\n\t"
-        /* ---- k > 0: run (1-bits add 2^k each), m zeros, then GR(k_R) + 1 ---- */
+        "s_branch 12b\n\t"
+        "13:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 14b\n\t"
         "30:\n\t"
+        /* ---- k > 0: run entry ---- */
         "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"        /* t = 1-bits of the run prefix */
-        "s_mov_b32 s82, 0\n\t"                     /* m */
-        "s_mov_b32 s87, %[kP]\n\t"                 /* k_P while the run is read */
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_mov_b32 s82, 0\n\t"
+        "s_mov_b32 s87, %[kP]\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
-        "s_cbranch_scc1 40f\n\t"
-        "s_lshr_b32 s85, s87, 2\n\t"               /* k */
-        "31:\n\t"
-        "s_add_u32 s86, s84, 1\n\t"                /* prefix length */
+        "s_cbranch_scc1 31f\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"
+        "32:\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
         "s_lshl_b64 s[80:81], %[win], s86\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"   /* Read(k), k >= 1 */
-        "s_add_u32 s82, s82, s90\n\t"              /* m */
-        "s_add_u32 s83, %[neg], s82\n\t"           /* carry iff the zeros reach the chunk end */
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_add_u32 s82, s82, s90\n\t"
+        "s_add_u32 s83, %[neg], s82\n\t"
         "s_cbranch_scc1 9f\n\t"
-        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"   /* after the run code */
-        "s_add_u32 s86, s86, s85\n\t"              /* run code length */
+        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
+        "s_add_u32 s86, s86, s85\n\t"
         "s_not_b64 s[80:81], s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"        /* GR prefix ones */
-        "s_lshr_b32 s89, %[kRP], 2\n\t"            /* k_R */
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_add_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_add_u32 s86, s86, 1\n\t"                /* total length */
+        "s_add_u32 s86, s86, 1\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 9f\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"   /* rem */
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
         "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"             /* SCC = ones != 0 */
+        "s_lshl_b32 s88, s84, s89\n\t"
         "s_cselect_b32 s91, -1, -2\n\t"
         "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"                /* u + 1 */
-        "s_sub_u32 %[kP], s87, 1\n\t"              /* k_P >= 4 here */
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_u32 %[kP], s87, 1\n\t"
         "s_add_i32 s90, %[kRP], s84\n\t"
         "s_add_i32 s90, s90, s91\n\t"
         "s_max_i32 s90, s90, 0\n\t"
@@ -256,25 +304,125 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
         "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 20b\n\t"
-        "s_branch 10b\n\t"
-        /* 1-bits in the run prefix: up to 6, k stays <= 24 */
-        "40:\n\t"
+        "s_cbranch_scc1 33f\n\t"
+        "34:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc0 10b\n\t"
+        /* ---- k > 0: run entry ---- */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_mov_b32 s82, 0\n\t"
+        "s_mov_b32 s87, %[kP]\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_cbranch_scc1 41f\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"
+        "42:\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
+        "s_lshl_b64 s[80:81], %[win], s86\n\t"
+        "s_sub_u32 s88, 64, s85\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_add_u32 s82, s82, s90\n\t"
+        "s_add_u32 s83, %[neg], s82\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
+        "s_add_u32 s86, s86, s85\n\t"
+        "s_not_b64 s[80:81], s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_add_u32 s86, s86, s84\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_add_u32 s86, s86, 1\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_cselect_b32 s91, -1, -2\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_u32 %[kP], s87, 1\n\t"
+        "s_add_i32 s90, %[kRP], s84\n\t"
+        "s_add_i32 s90, s90, s91\n\t"
+        "s_max_i32 s90, s90, 0\n\t"
+        "s_min_i32 %[kRP], s90, 128\n\t"
+        "s_mov_b32 %[neg], s83\n\t"
+        "v_mov_b32 %[vt], s88\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 43f\n\t"
+        "44:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc0 10b\n\t"
+        "s_branch 30b\n\t"
+        "33:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 34b\n\t"
+        "43:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 44b\n\t"
+        "31:\n\t"
         "s_cmp_gt_u32 s84, 6\n\t"
         "s_cbranch_scc1 9f\n\t"
         "s_sub_u32 s83, 0, s84\n\t"
-        "41:\n\t"
+        "35:\n\t"
         "s_lshr_b32 s85, s87, 2\n\t"
         "s_lshl_b32 s89, 1, s85\n\t"
         "s_add_u32 s82, s82, s89\n\t"
         "s_add_u32 s87, s87, 2\n\t"
         "s_add_u32 s83, s83, 1\n\t"
-        "s_cbranch_scc0 41b\n\t"
+        "s_cbranch_scc0 35b\n\t"
         "s_lshr_b32 s85, s87, 2\n\t"
         "s_cmp_gt_u32 s85, 24\n\t"
         "s_cbranch_scc1 9f\n\t"
-        "s_branch 31b\n\t"
+        "s_branch 32b\n\t"
+        "41:\n\t"
+        "s_cmp_gt_u32 s84, 6\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_sub_u32 s83, 0, s84\n\t"
+        "45:\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"
+        "s_lshl_b32 s89, 1, s85\n\t"
+        "s_add_u32 s82, s82, s89\n\t"
+        "s_add_u32 s87, s87, 2\n\t"
+        "s_add_u32 s83, s83, 1\n\t"
+        "s_cbranch_scc0 45b\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"
+        "s_cmp_gt_u32 s85, 24\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_branch 42b\n\t"
+        "7:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_branch 9f\n\t"
+        "8:\n\t"
         "9:\n\t"
+
         : [win] "+s"(win), [nbx] "+s"(nbx), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [neg] "+s"(neg),
           [acc] "+v"(acc), [vt] "=&v"(vt)
         : [cur] "v"(cur), [vrev] "v"(vrev)
