@@ -65,7 +65,7 @@ def measure_traffic(args) -> dict | None:
     me = [sys.executable, "-u", str(Path(__file__).resolve())]
     leg = me + ["--only", "batched", "--steps", "2", "--warmup", "1", "--no-cpu", "--no-traffic",
                 "--preset", args.preset, "--seed", str(args.seed)]
-    for k in ("waves", "lds", "wgs_per_cu", "queue_chunk"):
+    for k in ("waves", "lds", "wgs_per_cu", "queue_chunk", "priority_items"):
         v = getattr(args, k)
         if v:
             leg += ["--" + k.replace("_", "-"), str(v)]
@@ -162,6 +162,7 @@ def main() -> int:
     ap.add_argument("--lds", type=int, default=0)
     ap.add_argument("--wgs-per-cu", type=int, default=0)
     ap.add_argument("--queue-chunk", type=int, default=0)
+    ap.add_argument("--priority-items", type=int, default=0)
     ap.add_argument("--autotune", action="store_true", help="run the decoder autotuner on the first GoP before timing")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
     ap.add_argument("--no-serving", action="store_true", help="skip the multi-stream serving leg")
@@ -216,7 +217,7 @@ def main() -> int:
     run_p = args.only in ("all", "perframe")
     dec = pvvc.Decoder(n_patches, max_pts)
     tun = {k: v for k, v in dict(waves_per_patch=args.waves, lds_bytes=args.lds, wgs_per_cu=args.wgs_per_cu,
-                                 queue_chunk=args.queue_chunk).items() if v}
+                                 queue_chunk=args.queue_chunk, priority_items=args.priority_items).items() if v}
     if tun:
         dec.set_tuning(**tun)
     batches = [dec.batch(frames[g * gop:(g + 1) * gop]) for g in range(args.gops)]

@@ -173,6 +173,8 @@ typedef struct pvvc_tuning {
     int lds_bytes;         /* LDS workspace per work item (4..64 KiB) */
     int wgs_per_cu;        /* persistent workgroups per CU */
     int queue_chunk;       /* items taken per queue pop */
+    int priority_items;    /* the first N items of the length-sorted queue run at the highest
+                              wave priority (s_setprio 3); -1 = none */
 } pvvc_tuning;
 
 int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_patch);

@@ -1139,7 +1139,7 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
                                                               uint32_t* queue, SlotDev* slots, Sink sink,
                                                               uint32_t lds_bytes, uint8_t* slab, uint64_t slab_bytes,
                                                               uint32_t launch_id, int* __restrict__ status,
-                                                              float* stage, uint32_t chunk) {
+                                                              float* stage, uint32_t chunk, uint32_t prio_items) {
     __shared__ LevelMeta meta;
     __shared__ uint32_t next;
     uint8_t* ws = slab + static_cast<uint64_t>(blockIdx.x) * slab_bytes;
@@ -1158,7 +1158,12 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
             Arena ar{g_smem, lds_bytes, 0, ws, slab_bytes, 0};
             SliceArgs a = item_args(it);
             a.stage = stage;
+            /* the longest slices (queue head) win issue arbitration on their CU,
+             * so that the launch is not stretched by them running contended */
+            const bool hi = q < prio_items;
+            if (hi) __builtin_amdgcn_s_setprio(3);
             int st = decode_slice<WAVES>(a, slot, ar, sink, meta, launch_id, true, i);
+            if (hi) __builtin_amdgcn_s_setprio(0);
             if (threadIdx.x == 0) status[i] = st;
             __syncthreads();
         }
@@ -1544,7 +1549,7 @@ void fill_batch_image(const BatchPlan& p, const pvvc_frame* const* frames, uint8
 /* ---- native decoder / batch ---- */
 struct pvvc_decoder {
     StateInfo st;
-    pvvc_tuning tuning{1, 8 * 1024, 16, 1};
+    pvvc_tuning tuning{1, 8 * 1024, 16, 1, 0};
     uint32_t launch_id = 0;
     uint8_t* slab = nullptr;
     uint64_t slab_bytes = 0, slab_total = 0;
@@ -1588,12 +1593,12 @@ void launch_frame(const pvvc_batch* b, int f, const Sink& sink, uint32_t lds, hi
 
 template <int W>
 void launch_persistent_w(const BatchDev& b, const Segment& sg, const Sink& sink, uint32_t lds, uint32_t grid,
-                         uint32_t chunk, const LaunchRes& r, uint32_t launch_id, hipStream_t s) {
+                         uint32_t chunk, uint32_t prio, const LaunchRes& r, uint32_t launch_id, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, r.queue);
     auto ev = g_ktiming.next();
     hipEventRecord(ev.first, s);
     hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b.items, b.order + sg.begin, sg.count,
-                       r.queue, r.slots, sink, lds, r.slab, r.slab_bytes, launch_id, b.status, r.stage, chunk);
+                       r.queue, r.slots, sink, lds, r.slab, r.slab_bytes, launch_id, b.status, r.stage, chunk, prio);
     hipEventRecord(ev.second, s);
     if (sg.dcount)
         hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, b.items, b.defer + sg.dbegin, sg.dcount,
@@ -1642,13 +1647,13 @@ bool rt_warm_kernels(const pvvc_tuning& t, uint32_t* queue, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, queue);
     if (t.waves_per_patch == 2)
         hipLaunchKernelGGL(persistent_kernel<2>, dim3(1), dim3(128), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
-                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u);
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u);
     else if (t.waves_per_patch == 4)
         hipLaunchKernelGGL(persistent_kernel<4>, dim3(1), dim3(256), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
-                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u);
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u);
     else
         hipLaunchKernelGGL(persistent_kernel<1>, dim3(1), dim3(64), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
-                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u);
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u);
     return hip_ok(hipGetLastError()) && hip_ok(hipStreamSynchronize(s));
 }
 
@@ -1663,9 +1668,10 @@ void launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, cons
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(t.lds_bytes));
     const uint32_t grid = persistent_grid(t, sg.count);
     const uint32_t chunk = static_cast<uint32_t>(std::max(1, t.queue_chunk));
-    if (t.waves_per_patch == 2) launch_persistent_w<2>(b, sg, sink, lds, grid, chunk, r, launch_id, s);
-    else if (t.waves_per_patch == 4) launch_persistent_w<4>(b, sg, sink, lds, grid, chunk, r, launch_id, s);
-    else launch_persistent_w<1>(b, sg, sink, lds, grid, chunk, r, launch_id, s);
+    const uint32_t prio = static_cast<uint32_t>(std::max(0, t.priority_items));
+    if (t.waves_per_patch == 2) launch_persistent_w<2>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
+    else if (t.waves_per_patch == 4) launch_persistent_w<4>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
+    else launch_persistent_w<1>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
 }
 
 }  // namespace pvvc
@@ -2001,6 +2007,7 @@ int pvvc_decoder_set_tuning(pvvc_decoder* d, const pvvc_tuning* t) {
     if (t->lds_bytes) n.lds_bytes = t->lds_bytes;
     if (t->wgs_per_cu) n.wgs_per_cu = t->wgs_per_cu;
     if (t->queue_chunk) n.queue_chunk = t->queue_chunk;
+    if (t->priority_items) n.priority_items = t->priority_items;
     d->tuning = n;
     return PVVC_OK;
 }
