@@ -227,6 +227,28 @@ int pvvc_decoder_autotune(pvvc_decoder* dec, pvvc_batch* batch, int mode, int ou
                           Points* points, int reps, pvvc_tuning* best, double* best_ms);
 
 /* ------------------------------------------------------------------------ */
+/* Forward path: colour encoder (RAHTOctree::RAHT, raht_octree.cc:189-247;  */
+/* quantisation, GoP_encoding.cc:172-205; RLGREncoder::Encode,              */
+/* entropy_codec.cc:65-159) on the GPU, one wavefront per colour stream.     */
+/* ------------------------------------------------------------------------ */
+typedef struct pvvc_color_job {
+    const uint8_t* geometry;    /* host: the patch's octree bytes (as in an intra slice, unwrapped) */
+    uint32_t geometry_size;
+    const float* yuv;           /* host: n leaf colours, YUV AoS, in leaf (decoder output) order; the
+                                   reference averages the points of a leaf (raht_octree.cc:189-200) */
+    uint32_t n;
+    uint8_t qp;                 /* quantisation step (> 0) */
+    uint8_t* out;               /* host: receives the RLGR bytes (not zstd-wrapped) */
+    uint32_t out_cap;
+    uint32_t out_size;          /* result: bytes written */
+    int status;                 /* result: PVVC_OK, or BAD_SLICE / UNMATCHED_SIZE (geometry),
+                                   SYMBOL_RANGE (coefficient outside int32), CAPACITY (out_cap) */
+} pvvc_color_job;
+/* Encodes every job's colours (one launch); blocks until the bytes are back.
+ * kernel_ms (optional) receives the encode kernel's time. */
+int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms);
+
+/* ------------------------------------------------------------------------ */
 /* Serving loop: many concurrent streams, continuous batching.  The          */
 /* reference client decodes one stream with a loader thread and a decoder   */
 /* thread around launch_cudaProcess (src/cuda/common/manager.cc:145-310);    */

@@ -1214,6 +1214,335 @@ __global__ __launch_bounds__(256) void finish_kernel(const ItemDev* __restrict__
     }
 }
 
+/* ======================================================================== */
+/* Forward path: RAHT (raht_octree.cc:189-247) + quantisation               */
+/* (GoP_encoding.cc:172-205) + RLGR encoder (entropy_codec.cc:65-159)       */
+/* ======================================================================== */
+
+/* One colour stream to encode: the patch's octree bytes, its n leaf colours
+ * (YUV, leaf order = the decoder's output order) and the quantisation step. */
+struct EncJobDev {
+    const uint8_t* geo;
+    const float* yuv;      /* n x 3, AoS */
+    uint8_t* out;          /* out_cap bytes, 4-byte aligned */
+    uint8_t* ws;           /* workspace, ws_cap bytes */
+    uint64_t ws_cap;
+    uint32_t geo_size;
+    uint32_t n;
+    uint32_t out_cap;
+    uint32_t qp;
+};
+
+/* HierarchicalTransform (octree_base.cc:65-82) of one node: child DCs in
+ * iY/iU/iV (0 where absent), child weights cw; returns the DC and the coded
+ * ACs in NodeWeight order, FP32 in the reference's operation order
+ * (HaarTransform, octree_base.cc:164-184). */
+__device__ __forceinline__ int node_forward(const uint32_t cw[8], const float iY[8], const float iU[8],
+                                            const float iV[8], float& dY, float& dU, float& dV, float aY[7],
+                                            float aU[7], float aV[7]) {
+    uint32_t w[16];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[8 + i] = cw[i];
+    w[4] = w[8] + w[12];
+    w[5] = w[9] + w[13];
+    w[6] = w[10] + w[14];
+    w[7] = w[11] + w[15];
+    w[2] = w[4] + w[6];
+    w[3] = w[5] + w[7];
+    float rY[16], rU[16], rV[16], hY[8], hU[8], hV[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        rY[8 + i] = iY[i];
+        rU[8 + i] = iU[i];
+        rV[8 + i] = iV[i];
+    }
+    constexpr int NW[8][2] = {{0, 0}, {2, 3}, {4, 6}, {5, 7}, {8, 12}, {9, 13}, {10, 14}, {11, 15}};
+#pragma unroll
+    for (int i = 7; i >= 1; --i) {
+        const int a = NW[i][0], b = NW[i][1];
+        const uint32_t w0 = w[a], w1 = w[b];
+        if (w0 == 0 && w1 == 0) {
+            rY[i] = rU[i] = rV[i] = 0.0f;
+            hY[i] = hU[i] = hV[i] = 0.0f;
+            continue;
+        }
+        const float2 t = (w0 && w1 && w0 < kRahtTab && w1 < kRahtTab) ? g_raht_tab[w0 * kRahtTab + w1]
+                                                                       : haar_factors(w0, w1);
+        const float t00 = t.x, t01 = t.y, t10 = -t.y, t11 = t.x;
+        rY[i] = rY[a] * t00 + rY[b] * t01;
+        rU[i] = rU[a] * t00 + rU[b] * t01;
+        rV[i] = rV[a] * t00 + rV[b] * t01;
+        hY[i] = rY[a] * t10 + rY[b] * t11;
+        hU[i] = rU[a] * t10 + rU[b] * t11;
+        hV[i] = rV[a] * t10 + rV[b] * t11;
+    }
+    dY = rY[1];
+    dU = rU[1];
+    dV = rV[1];
+    int k = 0;
+#pragma unroll
+    for (int idx = 1; idx < 8; ++idx) {
+        if (w[NW[idx][0]] != 0 && w[NW[idx][1]] != 0) {
+            aY[k] = hY[idx];
+            aU[k] = hU[idx];
+            aV[k] = hV[idx];
+            ++k;
+        }
+    }
+    return k;
+}
+
+/* round(c / qp) (GoP_encoding.cc:174-181); false outside int32. */
+__device__ __forceinline__ bool quantise(float c, float qp, int32_t& q) {
+    const float r = roundf(c / qp);
+    if (!(r >= -2147483648.0f && r < 2147483648.0f)) return false;
+    q = static_cast<int32_t>(r);
+    return true;
+}
+
+/* MSB-first bit writer of one wavefront: whole 32-bit words are gathered in
+ * lane (word mod 64) of `stage` and stored 256 bytes at a time. */
+struct BitWriterW {
+    uint64_t buf;
+    uint32_t cnt;     /* pending bits, < 32 between calls */
+    uint32_t words;   /* words emitted */
+    uint32_t stage;
+    uint8_t* out;
+    uint32_t cap_words;
+    bool over;
+};
+
+__device__ __forceinline__ void bw_store_chunk(BitWriterW& w, uint32_t first_word, uint32_t count) {
+    const uint32_t lane = lane_id();
+    const uint32_t wi = first_word + lane;
+    if (lane < count) {
+        if (wi < w.cap_words) reinterpret_cast<uint32_t*>(w.out)[wi] = w.stage;
+    }
+    if (first_word + count > w.cap_words) w.over = true;
+}
+
+__device__ __forceinline__ void bw_word(BitWriterW& w, uint32_t word) {
+    const uint32_t lane = lane_id();
+    w.stage = (lane == (w.words & 63u)) ? __builtin_bswap32(word) : w.stage;
+    ++w.words;
+    if ((w.words & 63u) == 0) bw_store_chunk(w, w.words - 64u, 64u);
+}
+
+/* Write(x, bits) (entropy_codec.cc:22-36) for bits <= 32, x < 2^bits. */
+__device__ __forceinline__ void bw_put(BitWriterW& w, uint64_t x, uint32_t bits) {
+    if (bits == 0) return;
+    w.buf = (w.buf << bits) + x;
+    w.cnt += bits;
+    if (w.cnt >= 32) {
+        w.cnt -= 32;
+        bw_word(w, static_cast<uint32_t>(w.buf >> w.cnt));
+    }
+}
+
+/* GRWrite (entropy_codec.cc:38-63) */
+__device__ __forceinline__ void bw_gr(BitWriterW& w, uint64_t x, uint32_t k) {
+    const uint64_t p = x >> k;
+    if (p < 32) {
+        bw_put(w, (uint64_t{1} << (p + 1)) - 2, static_cast<uint32_t>(p) + 1);
+        bw_put(w, k >= 64 ? x : (x & ((uint64_t{1} << k) - 1)), k);
+    } else {
+        bw_put(w, 0xffffffffull, 32);
+        bw_put(w, x & 0xffffffffull, 32);
+    }
+}
+
+/* RLGR encode of sym[0, total) (entropy_codec.cc:65-159); one wavefront,
+ * wave-uniform control.  Returns the byte length, or -1 past out_cap. */
+__device__ __forceinline__ int64_t rlgr_encode_wave(const int32_t* __restrict__ sym, uint32_t total, uint8_t* out,
+                                                    uint32_t out_cap) {
+    const uint32_t lane = lane_id();
+    BitWriterW w{0, 0, 0, 0, out, out_cap / 4u, false};
+    uint64_t u = 0, kP = 0, kRP = 8, m = 0, k = 0, kR = 0;
+    for (uint32_t base = 0; base < total; base += 64u) {
+        const int32_t chunk = (base + lane < total) ? sym[base + lane] : 0;
+        const uint32_t lim = total - base < 64u ? total - base : 64u;
+        for (uint32_t j = 0; j < lim; ++j) {
+            const int64_t d = __builtin_amdgcn_readlane(chunk, static_cast<int>(j));
+            u = d < 0 ? (static_cast<uint64_t>(-d) << 1) - 1 : static_cast<uint64_t>(d) << 1;
+            k = kP >> 2;
+            kR = kRP >> 2;
+            if (k > 32) { /* needs > 2^32 zeros in a run; the decoder rejects k > 30 */
+                w.over = true;
+                return -1;
+            }
+            if (k) {
+                if (u) {
+                    --u;
+                    bw_put(w, 0, 1);
+                    bw_put(w, m, static_cast<uint32_t>(k));
+                    bw_gr(w, u, static_cast<uint32_t>(kR));
+                    const uint64_t p = u >> kR;
+                    if (p) {
+                        kRP += p - 1;
+                        kRP = kRP > 128 ? 128 : kRP;
+                    } else {
+                        kRP = kRP < 2 ? 0 : kRP - 2;
+                    }
+                    kP = kP < 1 ? 0 : kP - 1;
+                    m = 0;
+                } else if (++m == (uint64_t{1} << k)) {
+                    bw_put(w, 1, 1);
+                    kP += 2;
+                    m = 0;
+                }
+            } else {
+                bw_gr(w, u, static_cast<uint32_t>(kR));
+                const uint64_t p = u >> kR;
+                if (p) {
+                    kRP = kRP + p - 1 > 128 ? 128 : kRP + p - 1;
+                } else {
+                    kRP = kRP < 2 ? 0 : kRP - 2;
+                }
+                kP = u ? (kP < 1 ? 0 : kP - 1) : kP + 3;
+                m = 0;
+            }
+        }
+    }
+    if (total > 0 && k && !u) {
+        bw_put(w, 0, 1);
+        bw_put(w, m, static_cast<uint32_t>(kP >> 2));
+    }
+    const uint32_t r = w.cnt % 8u;
+    if (r) bw_put(w, 0, 8 - r);
+    const uint64_t bytes = 4ull * w.words + w.cnt / 8u;
+    if (w.cnt) { /* the last partial word, left aligned */
+        const uint32_t word = static_cast<uint32_t>(w.buf << (32u - w.cnt));
+        w.stage = (lane == (w.words & 63u)) ? __builtin_bswap32(word) : w.stage;
+        ++w.words;
+    }
+    if (w.words & 63u) bw_store_chunk(w, w.words & ~63u, w.words & 63u);
+    return w.over ? -1 : static_cast<int64_t>(bytes);
+}
+
+/* Encodes one colour stream with one wavefront.  Returns the byte length or
+ * -(status). */
+__device__ __forceinline__ int64_t encode_job(const EncJobDev& jb, Arena ar, LevelMeta& meta) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = jb.n;
+    if (!jb.geo || jb.geo_size < 25 || n == 0 || !jb.yuv || !jb.out) return -PVVC_ERR_BAD_SLICE;
+    const uint64_t imax = jb.geo_size - 25u;
+    int32_t* sym = static_cast<int32_t*>(ar.take(12ull * n));
+    uint32_t* fc = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
+    uint32_t* wt = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
+    if (!sym || !fc || !wt) return -PVVC_ERR_CAPACITY;
+    int ts = tree_structure(jb.geo, jb.geo_size, n, fc, meta);
+    __syncthreads();
+    if (ts != PVVC_OK) return -ts;
+    tree_weights(jb.geo, fc, wt, meta);
+    __syncthreads();
+    const uint32_t h = meta.height;
+    const float qp = static_cast<float>(jb.qp);
+    bool range_ok = true;
+    if (h == 1) { /* a single leaf: its colour is the DC */
+        if (tid == 0) {
+            int32_t q0, q1, q2;
+            range_ok = quantise(jb.yuv[0], qp, q0) && quantise(jb.yuv[1], qp, q1) && quantise(jb.yuv[2], qp, q2);
+            sym[0] = q0;
+            sym[1] = q1;
+            sym[2] = q2;
+        }
+    } else {
+        uint32_t wl = 1;
+        for (uint32_t l = 0; l + 1 < h; ++l) wl = meta.cnt[l] > wl ? meta.cnt[l] : wl;
+        float* bufA = static_cast<float*>(ar.take(12ull * wl));
+        float* bufB = static_cast<float*>(ar.take(12ull * wl));
+        if (!bufA || !bufB) return -PVVC_ERR_CAPACITY;
+        for (int l = static_cast<int>(h) - 2; l >= 0; --l) {
+            const uint32_t cnt = meta.cnt[l], base = meta.base[l], cbase = meta.base[l + 1];
+            const bool last = l == static_cast<int>(h) - 2;
+            const float* cin = ((h - 2 - l) & 1) ? bufA : bufB; /* children's DCs (level l+1) */
+            float* dout = ((h - 2 - l) & 1) ? bufB : bufA;
+            for (uint32_t j = tid; j < cnt; j += 64u) {
+                const uint32_t occ = jb.geo[25 + base + j];
+                const uint32_t f = fc[base + j];
+                uint32_t cw[8];
+                float iY[8], iU[8], iV[8];
+                uint32_t kk = 0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    iY[i] = iU[i] = iV[i] = 0.0f;
+                    cw[i] = 0;
+                    if ((occ >> (7 - i)) & 1u) {
+                        const uint32_t c = f + kk;
+                        ++kk;
+                        if (last) {
+                            cw[i] = 1;
+                            iY[i] = jb.yuv[3ull * c];
+                            iU[i] = jb.yuv[3ull * c + 1];
+                            iV[i] = jb.yuv[3ull * c + 2];
+                        } else {
+                            cw[i] = wt[cbase + c];
+                            iY[i] = cin[3 * c];
+                            iU[i] = cin[3 * c + 1];
+                            iV[i] = cin[3 * c + 2];
+                        }
+                    }
+                }
+                float dY, dU, dV, aY[7], aU[7], aV[7];
+                const int na = node_forward(cw, iY, iU, iV, dY, dU, dV, aY, aU, aV);
+                dout[3 * j] = dY;
+                dout[3 * j + 1] = dU;
+                dout[3 * j + 2] = dV;
+                /* the decoder reads node j's ACs at reversed positions cnt + f - j + t */
+                const uint32_t r0 = cnt + f - j;
+                for (int t = 0; t < na; ++t) {
+                    if (r0 + static_cast<uint32_t>(t) >= n) { /* cannot happen for a tree that passed */
+                        range_ok = false;                     /* tree_structure; never write outside */
+                        break;
+                    }
+                    const uint32_t pos = n - 1u - (r0 + static_cast<uint32_t>(t));
+                    int32_t q0, q1, q2;
+                    range_ok &= quantise(aY[t], qp, q0) && quantise(aU[t], qp, q1) && quantise(aV[t], qp, q2);
+                    sym[pos] = q0;
+                    sym[n + pos] = q1;
+                    sym[2 * n + pos] = q2;
+                }
+                if (l == 0) { /* the root's DC is the last coefficient */
+                    int32_t q0, q1, q2;
+                    range_ok &= quantise(dY, qp, q0) && quantise(dU, qp, q1) && quantise(dV, qp, q2);
+                    sym[n - 1] = q0;
+                    sym[2 * n - 1] = q1;
+                    sym[3 * n - 1] = q2;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (__ballot(!range_ok)) return -PVVC_ERR_SYMBOL_RANGE;
+    const int64_t bytes = rlgr_encode_wave(sym, 3u * n, jb.out, jb.out_cap);
+    return bytes < 0 ? -PVVC_ERR_CAPACITY : bytes;
+}
+
+/* Packs each job's bytes at its offset of a contiguous buffer. */
+__global__ __launch_bounds__(256) void gather_kernel(const EncJobDev* __restrict__ jobs, const int64_t* __restrict__ size,
+                                                     const uint64_t* __restrict__ at, uint32_t count, uint8_t* packed) {
+    const uint32_t i = blockIdx.x;
+    if (i >= count) return;
+    const int64_t n = size[i];
+    if (n <= 0) return;
+    const uint8_t* src = jobs[i].out;
+    uint8_t* dst = packed + at[i];
+    for (int64_t b = threadIdx.x; b < n; b += blockDim.x) dst[b] = src[b];
+}
+
+/* One wavefront per colour stream. */
+__global__ __launch_bounds__(64) void encode_kernel(const EncJobDev* __restrict__ jobs, uint32_t count,
+                                                    uint32_t lds_bytes, int64_t* __restrict__ result) {
+    __shared__ LevelMeta meta;
+    const uint32_t i = blockIdx.x;
+    if (i >= count) return;
+    const EncJobDev jb = jobs[i];
+    Arena ar{g_smem, lds_bytes, 0, jb.ws, jb.ws_cap, 0};
+    const int64_t r = encode_job(jb, ar, meta);
+    if (threadIdx.x == 0) result[i] = r;
+}
+
 }  // namespace pvvc
 
 /* ======================================================================== */
@@ -1318,9 +1647,9 @@ std::map<const void*, StateInfo> g_states;
 
 uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
-/* geometry capacity: 2 internal nodes per point + header; workspace: coef
+/* geometry capacity: up to ~8 internal nodes per point + header; workspace: coef
  * 12n + first-child / weight arrays over the geometry + two level buffers. */
-uint32_t geo_cap_for(int max_points) { return static_cast<uint32_t>(64 + 2ull * max_points + 25); }
+uint32_t geo_cap_for(int max_points) { return static_cast<uint32_t>(64 + 8ull * max_points + 25); }
 uint64_t ws_cap_for(int max_points) {
     uint64_t n = static_cast<uint64_t>(max_points);
     return align256(12 * n + 2 * (4 * (geo_cap_for(max_points) + 1ull) + 16) + 48 * n + 512);
@@ -2189,6 +2518,113 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
         hipEventRecord(ev.second, g_stream);
     }
     return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
+}
+
+/* ---- forward path (colour encoder) ---- */
+int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
+    if (!jobs || n_jobs <= 0) return PVVC_ERR_INVALID_ARG;
+    rt_init_kernels();
+    auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+    /* device image: [EncJobDev x n][results][offsets][per job: geo | yuv | out | ws] */
+    const uint64_t o_jobs = 0, o_res = al(sizeof(EncJobDev) * n_jobs), o_at = al(o_res + 8ull * n_jobs);
+    uint64_t at = al(o_at + 8ull * n_jobs);
+    std::vector<uint64_t> o_geo(n_jobs), o_yuv(n_jobs), o_out(n_jobs), o_ws(n_jobs), ws_cap(n_jobs), out_cap(n_jobs);
+    for (int i = 0; i < n_jobs; ++i) {
+        const pvvc_color_job& j = jobs[i];
+        if (!j.geometry || j.geometry_size < 25 || !j.yuv || j.n == 0 || !j.out || j.qp == 0)
+            return PVVC_ERR_INVALID_ARG;
+        const uint64_t n = j.n, imax = j.geometry_size - 25u;
+        o_geo[i] = at;
+        at = al(at + j.geometry_size);
+        o_yuv[i] = at;
+        at = al(at + 12 * n);
+        out_cap[i] = (std::min<uint64_t>(j.out_cap, 24 * n + 64) + 3) & ~uint64_t(3);
+        o_out[i] = at;
+        at = al(at + out_cap[i] + 4);
+        ws_cap[i] = al(12 * n + 2 * (4 * (imax + 1) + 16) + 24 * n + 512);
+        o_ws[i] = at;
+        at += ws_cap[i];
+    }
+    const uint64_t total = at;
+    uint8_t* dev = nullptr;
+    if (!hip_ok(hipMalloc(&dev, total))) return PVVC_ERR_HIP;
+    std::vector<uint8_t> h(o_at, 0);
+    std::vector<EncJobDev> jd(n_jobs);
+    for (int i = 0; i < n_jobs; ++i) {
+        EncJobDev& d = jd[i];
+        d.geo = dev + o_geo[i];
+        d.yuv = reinterpret_cast<const float*>(dev + o_yuv[i]);
+        d.out = dev + o_out[i];
+        d.ws = dev + o_ws[i];
+        d.ws_cap = ws_cap[i];
+        d.geo_size = jobs[i].geometry_size;
+        d.n = jobs[i].n;
+        d.out_cap = static_cast<uint32_t>(out_cap[i]);
+        d.qp = jobs[i].qp;
+    }
+    std::memcpy(h.data() + o_jobs, jd.data(), sizeof(EncJobDev) * n_jobs);
+    bool good = hip_ok(hipMemcpyAsync(dev, h.data(), h.size(), hipMemcpyHostToDevice, g_stream));
+    for (int i = 0; i < n_jobs && good; ++i) {
+        good = hip_ok(hipMemcpyAsync(dev + o_geo[i], jobs[i].geometry, jobs[i].geometry_size, hipMemcpyHostToDevice,
+                                     g_stream)) &&
+               hip_ok(hipMemcpyAsync(dev + o_yuv[i], jobs[i].yuv, 12ull * jobs[i].n, hipMemcpyHostToDevice, g_stream));
+    }
+    const uint32_t lds = static_cast<uint32_t>(clamp_lds(16 * 1024));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    std::vector<int64_t> res(n_jobs, 0);
+    if (good) {
+        hipEventRecord(e0, g_stream);
+        hipLaunchKernelGGL(encode_kernel, dim3(n_jobs), dim3(64), lds, g_stream,
+                           reinterpret_cast<const EncJobDev*>(dev + o_jobs), static_cast<uint32_t>(n_jobs), lds,
+                           reinterpret_cast<int64_t*>(dev + o_res));
+        hipEventRecord(e1, g_stream);
+        good = hip_ok(hipGetLastError()) &&
+               hip_ok(hipMemcpyAsync(res.data(), dev + o_res, 8ull * n_jobs, hipMemcpyDeviceToHost, g_stream)) &&
+               hip_ok(hipStreamSynchronize(g_stream));
+    }
+    if (good && kernel_ms) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        *kernel_ms = ms;
+    }
+    /* compact the variable-length outputs, one copy back */
+    std::vector<uint64_t> pos(n_jobs, 0);
+    uint64_t packed_bytes = 0;
+    for (int i = 0; i < n_jobs; ++i) {
+        pos[i] = packed_bytes;
+        if (res[i] > 0) packed_bytes += static_cast<uint64_t>(res[i]);
+    }
+    uint8_t* packed = nullptr;
+    if (good && packed_bytes) {
+        good = hip_ok(hipMalloc(&packed, packed_bytes)) &&
+               hip_ok(hipMemcpyAsync(dev + o_at, pos.data(), 8ull * n_jobs, hipMemcpyHostToDevice, g_stream));
+        if (good) {
+            hipLaunchKernelGGL(gather_kernel, dim3(n_jobs), dim3(256), 0, g_stream,
+                               reinterpret_cast<const EncJobDev*>(dev + o_jobs),
+                               reinterpret_cast<const int64_t*>(dev + o_res),
+                               reinterpret_cast<const uint64_t*>(dev + o_at), static_cast<uint32_t>(n_jobs), packed);
+            std::vector<uint8_t> hp(packed_bytes);
+            good = hip_ok(hipGetLastError()) &&
+                   hip_ok(hipMemcpyAsync(hp.data(), packed, packed_bytes, hipMemcpyDeviceToHost, g_stream)) &&
+                   hip_ok(hipStreamSynchronize(g_stream));
+            for (int i = 0; good && i < n_jobs; ++i)
+                if (res[i] > 0) {
+                    if (static_cast<uint64_t>(res[i]) > jobs[i].out_cap) res[i] = -PVVC_ERR_CAPACITY;
+                    else std::memcpy(jobs[i].out, hp.data() + pos[i], static_cast<size_t>(res[i]));
+                }
+        }
+    }
+    for (int i = 0; i < n_jobs; ++i) {
+        jobs[i].out_size = res[i] > 0 ? static_cast<uint32_t>(res[i]) : 0u;
+        jobs[i].status = !good ? PVVC_ERR_HIP : (res[i] > 0 ? PVVC_OK : static_cast<int>(-res[i]));
+    }
+    if (packed) hipFree(packed);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(dev);
+    return good ? PVVC_OK : PVVC_ERR_HIP;
 }
 
 /* Autotuner: coordinate descent over the launch knobs on a real batch.

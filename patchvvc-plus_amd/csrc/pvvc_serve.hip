@@ -526,7 +526,7 @@ int pvvc_server_create(const pvvc_server_config* cfg_in, pvvc_server** out) {
     bool good = ok(hipMalloc(&sv->slots, sizeof(SlotDev) * cfg.max_slots)) &&
                 ok(hipMalloc(&sv->pool_xyz, 12ull * cfg.out_pool_points)) &&
                 ok(hipMalloc(&sv->pool_rgb, 3ull * cfg.out_pool_points));
-    const uint32_t max_geo = static_cast<uint32_t>(64 + 2ull * cfg.max_points_per_patch + 25);
+    const uint32_t max_geo = static_cast<uint32_t>(64 + 8ull * cfg.max_points_per_patch + 25);
     const uint64_t need = al256(item_ws_bound(static_cast<uint32_t>(cfg.max_points_per_patch), max_geo));
     const uint32_t grid = persistent_grid(cfg.tuning, 0xffffffffu);
     sv->lanes = std::vector<Lane>(cfg.lanes);
@@ -605,7 +605,7 @@ int pvvc_server_open_stream(pvvc_server* sv, int max_patches, int max_points_per
     st.max_pts_patch = max_points_per_patch;
     st.max_pts_frame = max_points_per_frame;
     st.ring = out_ring;
-    st.geo_cap = static_cast<uint32_t>(64 + 2ull * max_points_per_patch + 25);
+    st.geo_cap = static_cast<uint32_t>(64 + 8ull * max_points_per_patch + 25);
     st.footprint = stream_state_bytes(max_patches, max_points_per_patch, st.geo_cap) +
                    15ull * static_cast<uint64_t>(max_points_per_frame) * out_ring;
     if (st.footprint > sv->budget) return PVVC_ERR_CAPACITY;

@@ -864,6 +864,50 @@ int64_t pvvc_enc_rlgr(const int64_t* sym, int64_t n, uint8_t* out, int64_t cap) 
     return static_cast<int64_t>(w.out.size());
 }
 
+/* One patch's colour stream on the CPU, the reference for the GPU encoder:
+ * builds the octree of xyz (n points), writes its bytes to geo_out, the
+ * per-leaf YUV (points of a leaf averaged, raht_octree.cc:189-200) in leaf
+ * order to leaf_yuv (leaves x 3), and the RLGR bytes of the quantised RAHT
+ * coefficients (no zstd) to col_out.  Returns the colour byte count, or -1;
+ * *leaves and *geo_size receive the leaf count and geometry size. */
+int64_t pvvc_enc_patch_colors(const float* xyz, int n, const float* yuv, int qp, uint8_t* geo_out, int64_t geo_cap,
+                              int64_t* geo_size, float* leaf_yuv, int* leaves, uint8_t* col_out, int64_t col_cap) {
+    if (n <= 0 || qp <= 0) return -1;
+    std::vector<XYZ> p(n);
+    std::vector<YUV> c(n);
+    for (int i = 0; i < n; ++i) {
+        p[i] = XYZ{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        c[i] = YUV{yuv[3 * i], yuv[3 * i + 1], yuv[3 * i + 2]};
+    }
+    RahtOctree t;
+    if (!t.make_tree(p.data(), n)) return -1;
+    std::vector<uint8_t> geo = t.get_octree();
+    if (geo_size) *geo_size = static_cast<int64_t>(geo.size());
+    if (geo_out) std::memcpy(geo_out, geo.data(), std::min<size_t>(geo.size(), static_cast<size_t>(geo_cap)));
+    const size_t L = t.leaves();
+    if (leaves) *leaves = static_cast<int>(L);
+    if (leaf_yuv) {
+        size_t k = 0;
+        for (auto& nd : t.tree.back()) {
+            YUV a;
+            for (int i : nd.index) {
+                a.y += c[i].y;
+                a.u += c[i].u;
+                a.v += c[i].v;
+            }
+            const int m = static_cast<int>(nd.index.size());
+            leaf_yuv[3 * k] = a.y / m;
+            leaf_yuv[3 * k + 1] = a.u / m;
+            leaf_yuv[3 * k + 2] = a.v / m;
+            ++k;
+        }
+    }
+    Slice sl;
+    encode_colors(t, c.data(), static_cast<uint8_t>(qp), -1000, sl);
+    if (col_out) std::memcpy(col_out, sl.col.data(), std::min<size_t>(sl.col.size(), static_cast<size_t>(col_cap)));
+    return static_cast<int64_t>(sl.col.size());
+}
+
 /* Encodes one GoP of one patch: common geometry xyz (n x 3), RGB8 colours
  * for F frames (F x n x 3), F motion matrices (row-major 4x4); writes F
  * slices into a frame-image-ready form via callbacks of the caller:

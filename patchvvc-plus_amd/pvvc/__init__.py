@@ -424,6 +424,47 @@ class Decoder:
 
 
 # ---------------------------------------------------------------------------
+# Forward path (colour encoder)
+# ---------------------------------------------------------------------------
+def encode_colors(jobs: Sequence[tuple[bytes, np.ndarray, int]]) -> tuple[list[bytes], list[int], float]:
+    """GPU colour encoder: jobs of (octree bytes, leaf YUV float32 [n, 3], qp).
+    Returns (RLGR byte strings, statuses, kernel ms)."""
+    from ._native import ColorJob
+    n = len(jobs)
+    arr = (ColorJob * n)()
+    keep = []
+    for i, (geo, yuv, qp) in enumerate(jobs):
+        g = np.frombuffer(bytes(geo), np.uint8).copy()
+        y = np.ascontiguousarray(yuv, np.float32).reshape(-1, 3)
+        cap = 24 * len(y) + 64
+        o = np.zeros(cap, np.uint8)
+        keep += [g, y, o]
+        arr[i] = ColorJob(g.ctypes.data, g.size, y.ctypes.data, len(y), int(qp), o.ctypes.data, cap, 0, 0)
+    ms = C.c_double()
+    _check(hip_lib().pvvc_encode_colors(arr, n, C.byref(ms)), "pvvc_encode_colors")
+    outs = [keep[3 * i + 2][: arr[i].out_size].tobytes() for i in range(n)]
+    return outs, [arr[i].status for i in range(n)], float(ms.value)
+
+
+def encode_colors_cpu(xyz: np.ndarray, yuv: np.ndarray, qp: int) -> tuple[bytes, np.ndarray, bytes]:
+    """The stream writer's CPU encoder of one patch: (octree bytes, leaf YUV, RLGR bytes)."""
+    L = enc_lib()
+    p = np.ascontiguousarray(xyz, np.float32)
+    c = np.ascontiguousarray(yuv, np.float32)
+    n = len(p)
+    geo = np.zeros(64 * n + 128, np.uint8)  # internal nodes <= (height - 1) x n
+    gsz = C.c_int64()
+    leaf = np.zeros((n, 3), np.float32)
+    nl = C.c_int()
+    col = np.zeros(24 * n + 64, np.uint8)
+    r = L.pvvc_enc_patch_colors(p.ctypes.data, n, c.ctypes.data, int(qp), geo.ctypes.data, geo.size, C.byref(gsz),
+                                leaf.ctypes.data, C.byref(nl), col.ctypes.data, col.size)
+    if r < 0:
+        raise PvvcError(1, "pvvc_enc_patch_colors")
+    return geo[: gsz.value].tobytes(), leaf[: nl.value], col[:r].tobytes()
+
+
+# ---------------------------------------------------------------------------
 # Serving loop
 # ---------------------------------------------------------------------------
 STREAM_WAITING, STREAM_ACTIVE, STREAM_CLOSING, STREAM_CLOSED = 0, 1, 2, 3

@@ -56,6 +56,12 @@ class ServerStats(C.Structure):
                 ("backlog_frames", C.c_int64)]
 
 
+class ColorJob(C.Structure):
+    _fields_ = [("geometry", c_vp), ("geometry_size", C.c_uint32), ("yuv", c_vp), ("n", C.c_uint32),
+                ("qp", C.c_uint8), ("out", c_vp), ("out_cap", C.c_uint32), ("out_size", C.c_uint32),
+                ("status", C.c_int)]
+
+
 class SynthConfig(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("grid_bits", "n_frames", "gop", "qp_i", "qp_p", "n_humanoids", "jitter",
                                        "patch_min", "patch_max", "zstd_level", "threads")] + \
@@ -124,6 +130,7 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_batch_state_bytes", C.c_int64, c_vp)
     _sig(L, "pvvc_batch_status", C.c_int, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_decode_batch", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp)
+    _sig(L, "pvvc_encode_colors", C.c_int, C.POINTER(ColorJob), C.c_int, C.POINTER(C.c_double))
     _sig(L, "pvvc_server_clock", C.c_double)
     _sig(L, "pvvc_server_create", C.c_int, C.POINTER(ServerConfig), C.POINTER(c_vp))
     _sig(L, "pvvc_server_destroy", C.c_int, c_vp)
@@ -168,6 +175,8 @@ def enc_lib() -> C.CDLL:
     _sig(L, "pvvc_enc_octree", C.c_int64, c_vp, C.c_int, c_vp, C.c_int64, c_i32p)
     _sig(L, "pvvc_enc_raht", C.c_int64, c_vp, C.c_int, c_vp, c_vp)
     _sig(L, "pvvc_enc_rlgr", C.c_int64, c_vp, C.c_int64, c_vp, C.c_int64)
+    _sig(L, "pvvc_enc_patch_colors", C.c_int64, c_vp, C.c_int, c_vp, C.c_int, c_vp, C.c_int64, c_i64p, c_vp, c_i32p,
+         c_vp, C.c_int64)
     _sig(L, "pvvc_enc_gop_patch", C.c_int64, c_vp, C.c_int, c_vp, C.c_int, C.c_int, C.c_int, C.c_int, c_vp,
          C.c_int64)
     _enc = L

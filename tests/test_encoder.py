@@ -1,0 +1,76 @@
+"""Forward path: the GPU colour encoder (forward RAHT, quantisation, RLGR
+writer) against the stream writer's CPU encoder, byte for byte, on
+surface-like voxel patches with intra-like and residual-like colours."""
+import numpy as np
+import pytest
+
+
+def _surface_patch(rng, n_target):
+    """Unique voxels near a random sphere cap: a dense surface like a patch."""
+    r = rng.uniform(8, 40)
+    pts = set()
+    c = rng.uniform(50, 200, 3)
+    tries = 0
+    while len(pts) < n_target and tries < 20 * n_target:
+        tries += 1
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        if v[2] < 0.2:
+            continue
+        p = np.floor(c + r * v + rng.normal(0, 0.3, 3))
+        pts.add((p[0], p[1], p[2]))
+    return np.array(sorted(pts), np.float32)
+
+
+def _jobs(pvvc, seed=3, count=40):
+    rng = np.random.default_rng(seed)
+    jobs, refs = [], []
+    sizes = [1, 2, 3, 9] + list(rng.integers(10, 3000, count - 4))
+    for t, n in enumerate(sizes):
+        xyz = _surface_patch(rng, int(n)) if n > 3 else np.array([[5, 7, 9], [6, 7, 9], [5, 8, 9]][: n],
+                                                                    np.float32)
+        m = len(xyz)
+        base = 128 + 60 * np.sin(xyz @ rng.normal(0, 0.05, (3, 3)))
+        yuv = (base + rng.normal(0, 4, (m, 3))).astype(np.float32)
+        if t % 3 == 2:  # residual-like: signed, small
+            yuv = rng.normal(0, 6, (m, 3)).astype(np.float32)
+        qp = int(rng.choice([1, 2, 4, 8]))
+        geo, leaf, col = pvvc.encode_colors_cpu(xyz, yuv, qp)
+        # leaves may hold several points (the tree's range is max - min, so
+        # neighbouring voxels can share a leaf); the encoders take leaf colours
+        assert 1 <= len(leaf) <= m
+        jobs.append((geo, leaf, qp))
+        refs.append(col)
+    return jobs, refs
+
+
+def test_cpu_encoder_decodes_with_oracle(pvvc, oracle):
+    """The CPU reference encoder's stream decodes (oracle) to the RAHT of the
+    leaf colours within the quantisation step."""
+    jobs, refs = _jobs(pvvc, seed=5, count=8)
+    for (geo, leaf, qp), col in zip(jobs, refs):
+        p = oracle.Patch()
+        assert p.set_slice(0x01, len(leaf), qp, geo, col) == 0
+        out = p.get(len(leaf))
+        # RAHT is orthonormal: per-leaf error is bounded by the per-coefficient error
+        err = np.abs(out["yuv"].astype(np.float64) - leaf.astype(np.float64)).max()
+        assert err <= qp * np.sqrt(3 * len(leaf)) / 2 + 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_encoder_matches_cpu(pvvc):
+    jobs, refs = _jobs(pvvc)
+    outs, status, ms = pvvc.encode_colors(jobs)
+    assert status == [0] * len(jobs), status
+    for i, (o, r) in enumerate(zip(outs, refs)):
+        assert o == r, f"job {i}: n={len(jobs[i][1])} qp={jobs[i][2]} {len(o)} vs {len(r)} bytes"
+    assert ms > 0
+
+
+@pytest.mark.gpu
+def test_gpu_encoder_rejects_bad_geometry(pvvc):
+    geo, leaf, qp = _jobs(pvvc, seed=9, count=5)[0][4]
+    bad = bytearray(geo)
+    bad[24] = 0  # height 0
+    outs, status, _ = pvvc.encode_colors([(bytes(bad), leaf, qp), (geo, leaf[:-1], qp)])
+    assert status == [1, 3]  # BAD_SLICE, UNMATCHED_SIZE (leaf count != n)
