@@ -166,6 +166,7 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true", help="run the decoder autotuner on the first GoP before timing")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
     ap.add_argument("--no-serving", action="store_true", help="skip the multi-stream serving leg")
+    ap.add_argument("--no-encode", action="store_true", help="skip the GPU colour-encoder leg")
     ap.add_argument("--serve-streams", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-lanes", type=int, default=3)
@@ -340,6 +341,63 @@ def main() -> int:
             st_.close()
         log("end to end", e2e)
 
+    # ---------------- forward path: GPU colour encoder ----------------
+    # Re-encodes the GoP's colours: frame 0's octree per patch, leaf YUV of the
+    # decoded frame (predictive frames: the residual against frame 0), the
+    # stream's qp.  The CPU leg is the stream writer's encoder on the same
+    # octrees (RAHT + RLGR, one thread) over frame 0, whose bytes the GPU
+    # output must equal.
+    encode = None
+    if run_b and not args.no_encode:
+        _, rgb_g = dec.decode_host(batches[0], pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+        rgbf = rgb_g.astype(np.float32)
+        r_, g_, b_ = rgbf[:, 0], rgbf[:, 1], rgbf[:, 2]
+        yuv = np.stack([r_ * np.float32(0.299) + g_ * np.float32(0.587) + b_ * np.float32(0.114),
+                        r_ * np.float32(-0.168736) - g_ * np.float32(0.331264) + b_ * np.float32(0.5) + np.float32(128),
+                        r_ * np.float32(0.5) - g_ * np.float32(0.418688) - b_ * np.float32(0.081312) + np.float32(128)],
+                       1).astype(np.float32)
+        m0 = metas[0]
+        geo_of = {int(m0["index"][s_]): frames[0].geometry(s_) for s_ in range(frames[0].slice_cnt)}
+        base = {}
+        o = 0
+        for s_ in range(frames[0].slice_cnt):
+            base[int(m0["index"][s_])] = (o, int(m0["size"][s_]))
+            o += int(m0["size"][s_])
+        jobs = []
+        for f in range(gop):
+            fo = batches[0].frame_offset[f]
+            mf = metas[f]
+            o = 0
+            for s_ in range(frames[f].slice_cnt):
+                p_, n_ = int(mf["index"][s_]), int(mf["size"][s_])
+                leaf = yuv[fo + o: fo + o + n_]
+                if f:
+                    b0, n0 = base[p_]
+                    leaf = leaf - yuv[b0: b0 + n0]
+                jobs.append((geo_of[p_], leaf, int(mf["qp"][s_])))
+                o += n_
+        pvvc.encode_colors(jobs[:64])  # warm
+        t_e = time.perf_counter()
+        outs_e, st_e, ms_e = pvvc.encode_colors(jobs)
+        wall_e = time.perf_counter() - t_e
+        pts_e = sum(len(j[1]) for j in jobs)
+        ns0 = frames[0].slice_cnt
+        t_c = time.perf_counter()
+        equal, cpu_pts = 0, 0
+        for s_ in range(ns0):
+            geo_j, leaf_j, qp_j = jobs[s_]
+            col = pvvc.encode_colors_cpu_on_tree(geo_j, leaf_j, qp_j)
+            equal += col == outs_e[s_]
+            cpu_pts += len(leaf_j)
+        cpu_s = time.perf_counter() - t_c
+        encode = {"streams": len(jobs), "points": pts_e, "kernel_ms": round(ms_e, 3),
+                  "gpu_mpts_per_s_kernel": round(pts_e / (ms_e * 1e-3) / 1e6, 1),
+                  "gpu_mpts_per_s_with_copies": round(pts_e / wall_e / 1e6, 1),
+                  "status_ok": int(sum(1 for x in st_e if x == 0)),
+                  "cpu_writer_mpts_per_s_1thread": round(cpu_pts / cpu_s / 1e6, 2),
+                  "frame0_streams_equal_to_cpu": f"{equal}/{ns0}"}
+        log("encode", encode)
+
     # ---------------- serving loop: many concurrent streams ----------------
     serving = None
     if not args.no_serving and args.only == "all":
@@ -388,7 +446,7 @@ def main() -> int:
             # colour accuracy of the batched GPU output against the FP64 inverse-RAHT
             # oracle on the first two frames of the GoP (one intra, one predictive)
             if run_b:
-                xyz_g, rgb_g = dec.decode_host(batches[0], pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+                _, rgb_g = dec.decode_host(batches[0], pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
                 o64 = binding.StreamDecoder(fp64=True)
                 diffs, exact, geo_ok, cnt = [], 0, True, 0
                 for k in range(min(2, gop)):
@@ -438,12 +496,24 @@ def main() -> int:
         "state_bytes_per_step": int(np.mean(state_bytes)),
         "rlgr_symbols_per_s_per_cu": round(sym_per_s_cu, 1),
         "end_to_end_per_frame": e2e,
+        "encode": encode,
         "accuracy": accuracy,
         "serving": serving,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(res))
+    print(json.dumps(_finite(res), allow_nan=False))
     return 0
+
+
+def _finite(x):
+    """Legs that did not run leave NaN; JSON has no NaN, so they become null."""
+    if isinstance(x, dict):
+        return {k: _finite(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_finite(v) for v in x]
+    if isinstance(x, float) and not np.isfinite(x):
+        return None
+    return x
 
 
 if __name__ == "__main__":

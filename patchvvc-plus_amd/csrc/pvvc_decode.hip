@@ -1410,12 +1410,18 @@ __device__ __forceinline__ int64_t rlgr_encode_wave(const int32_t* __restrict__ 
     const uint32_t r = w.cnt % 8u;
     if (r) bw_put(w, 0, 8 - r);
     const uint64_t bytes = 4ull * w.words + w.cnt / 8u;
-    if (w.cnt) { /* the last partial word, left aligned */
+    /* words of the open chunk not yet stored (a full chunk was stored by bw_word) */
+    bool pending = (w.words & 63u) != 0;
+    if (w.cnt) { /* the last partial word, left aligned; it may complete the chunk */
         const uint32_t word = static_cast<uint32_t>(w.buf << (32u - w.cnt));
         w.stage = (lane == (w.words & 63u)) ? __builtin_bswap32(word) : w.stage;
         ++w.words;
+        pending = true;
     }
-    if (w.words & 63u) bw_store_chunk(w, w.words & ~63u, w.words & 63u);
+    if (pending) {
+        const uint32_t first = (w.words - 1u) & ~63u;
+        bw_store_chunk(w, first, w.words - first);
+    }
     return w.over ? -1 : static_cast<int64_t>(bytes);
 }
 
@@ -2525,20 +2531,24 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     if (!jobs || n_jobs <= 0) return PVVC_ERR_INVALID_ARG;
     rt_init_kernels();
     auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
-    /* device image: [EncJobDev x n][results][offsets][per job: geo | yuv | out | ws] */
+    /* device image: [EncJobDev x n][results][offsets][inputs: geo | yuv per job][outputs and workspaces] */
     const uint64_t o_jobs = 0, o_res = al(sizeof(EncJobDev) * n_jobs), o_at = al(o_res + 8ull * n_jobs);
     uint64_t at = al(o_at + 8ull * n_jobs);
     std::vector<uint64_t> o_geo(n_jobs), o_yuv(n_jobs), o_out(n_jobs), o_ws(n_jobs), ws_cap(n_jobs), out_cap(n_jobs);
+    /* inputs (geometry, leaf YUV) of all jobs first, so that one copy covers them */
     for (int i = 0; i < n_jobs; ++i) {
         const pvvc_color_job& j = jobs[i];
         if (!j.geometry || j.geometry_size < 25 || !j.yuv || j.n == 0 || !j.out || j.qp == 0)
             return PVVC_ERR_INVALID_ARG;
-        const uint64_t n = j.n, imax = j.geometry_size - 25u;
         o_geo[i] = at;
         at = al(at + j.geometry_size);
         o_yuv[i] = at;
-        at = al(at + 12 * n);
-        out_cap[i] = (std::min<uint64_t>(j.out_cap, 24 * n + 64) + 3) & ~uint64_t(3);
+        at = al(at + 12ull * j.n);
+    }
+    const uint64_t in_end = at;
+    for (int i = 0; i < n_jobs; ++i) {
+        const uint64_t n = jobs[i].n, imax = jobs[i].geometry_size - 25u;
+        out_cap[i] = (std::min<uint64_t>(jobs[i].out_cap, 24 * n + 64) + 3) & ~uint64_t(3);
         o_out[i] = at;
         at = al(at + out_cap[i] + 4);
         ws_cap[i] = al(12 * n + 2 * (4 * (imax + 1) + 16) + 24 * n + 512);
@@ -2548,7 +2558,8 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     const uint64_t total = at;
     uint8_t* dev = nullptr;
     if (!hip_ok(hipMalloc(&dev, total))) return PVVC_ERR_HIP;
-    std::vector<uint8_t> h(o_at, 0);
+    /* [jobs][results][offsets][inputs]: one host image, one copy */
+    std::vector<uint8_t> h(in_end, 0);
     std::vector<EncJobDev> jd(n_jobs);
     for (int i = 0; i < n_jobs; ++i) {
         EncJobDev& d = jd[i];
@@ -2563,12 +2574,11 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
         d.qp = jobs[i].qp;
     }
     std::memcpy(h.data() + o_jobs, jd.data(), sizeof(EncJobDev) * n_jobs);
-    bool good = hip_ok(hipMemcpyAsync(dev, h.data(), h.size(), hipMemcpyHostToDevice, g_stream));
-    for (int i = 0; i < n_jobs && good; ++i) {
-        good = hip_ok(hipMemcpyAsync(dev + o_geo[i], jobs[i].geometry, jobs[i].geometry_size, hipMemcpyHostToDevice,
-                                     g_stream)) &&
-               hip_ok(hipMemcpyAsync(dev + o_yuv[i], jobs[i].yuv, 12ull * jobs[i].n, hipMemcpyHostToDevice, g_stream));
+    for (int i = 0; i < n_jobs; ++i) {
+        std::memcpy(h.data() + o_geo[i], jobs[i].geometry, jobs[i].geometry_size);
+        std::memcpy(h.data() + o_yuv[i], jobs[i].yuv, 12ull * jobs[i].n);
     }
+    bool good = hip_ok(hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice));
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(16 * 1024));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEventCreate(&e0);

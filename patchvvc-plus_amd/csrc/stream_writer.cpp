@@ -908,6 +908,54 @@ int64_t pvvc_enc_patch_colors(const float* xyz, int n, const float* yuv, int qp,
     return static_cast<int64_t>(sl.col.size());
 }
 
+/* The CPU colour encoder on a given octree (its bytes, as in an intra slice):
+ * the tree is rebuilt from the occupancy bytes (LoadTreeCore / MakeTree of
+ * the decoder, invert_raht_octree.cc:122-254), leaf k holding point k of
+ * leaf_yuv (n leaves); then RAHT, quantisation and RLGR as encode_colors.
+ * Returns the colour byte count, or -1 (bad geometry / leaf count != n). */
+int64_t pvvc_enc_colors_on_tree(const uint8_t* geo, int64_t geo_size, const float* leaf_yuv, int n, int qp,
+                                uint8_t* col_out, int64_t col_cap) {
+    if (!geo || geo_size < 25 || n <= 0 || qp <= 0) return -1;
+    RahtOctree t;
+    t.height = geo[24];
+    if (t.height <= 0 || t.height > 64) return -1;
+    t.tree.assign(t.height, {});
+    t.tree[0].emplace_back();
+    size_t at = 25;
+    for (int l = 0; l + 1 < t.height; ++l) {
+        for (auto& nd : t.tree[l]) {
+            if (at >= static_cast<size_t>(geo_size)) return -1;
+            nd.value = geo[at++];
+            if (nd.value == 0) return -1;
+            nd.index.assign(8, -1);
+            for (int i = 0; i < 8; ++i)
+                if (nd.value & kNodeValue[i]) {
+                    nd.index[i] = static_cast<int>(t.tree[l + 1].size());
+                    t.tree[l + 1].emplace_back();
+                }
+        }
+    }
+    if (static_cast<int>(t.tree.back().size()) != n) return -1;
+    for (int k = 0; k < n; ++k) {
+        FNode& lf = t.tree.back()[k];
+        lf.value = 0xff;
+        lf.index.assign(1, k);
+        lf.weight[1] = 1;
+    }
+    for (int l = t.height - 2; l >= 0; --l)
+        for (auto& nd : t.tree[l]) {
+            for (int i = 0; i < 8; ++i)
+                if (nd.index[i] != -1) nd.weight[i + 8] = t.tree[l + 1][nd.index[i]].weight[1];
+            for (int i = 7; i > 0; --i) nd.weight[i] = nd.weight[kNodeWeight[i][0]] + nd.weight[kNodeWeight[i][1]];
+        }
+    std::vector<YUV> c(n);
+    for (int k = 0; k < n; ++k) c[k] = YUV{leaf_yuv[3 * k], leaf_yuv[3 * k + 1], leaf_yuv[3 * k + 2]};
+    Slice sl;
+    encode_colors(t, c.data(), static_cast<uint8_t>(qp), -1000, sl);
+    if (col_out) std::memcpy(col_out, sl.col.data(), std::min<size_t>(sl.col.size(), static_cast<size_t>(col_cap)));
+    return static_cast<int64_t>(sl.col.size());
+}
+
 /* Encodes one GoP of one patch: common geometry xyz (n x 3), RGB8 colours
  * for F frames (F x n x 3), F motion matrices (row-major 4x4); writes F
  * slices into a frame-image-ready form via callbacks of the caller:

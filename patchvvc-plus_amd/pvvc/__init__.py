@@ -446,6 +446,18 @@ def encode_colors(jobs: Sequence[tuple[bytes, np.ndarray, int]]) -> tuple[list[b
     return outs, [arr[i].status for i in range(n)], float(ms.value)
 
 
+def encode_colors_cpu_on_tree(geo: bytes, leaf_yuv: np.ndarray, qp: int) -> bytes:
+    """The stream writer's CPU encoder on a given octree (bytes) and leaf YUV."""
+    L = enc_lib()
+    g = np.frombuffer(bytes(geo), np.uint8).copy()
+    y = np.ascontiguousarray(leaf_yuv, np.float32).reshape(-1, 3)
+    out = np.zeros(24 * len(y) + 64, np.uint8)
+    r = L.pvvc_enc_colors_on_tree(g.ctypes.data, g.size, y.ctypes.data, len(y), int(qp), out.ctypes.data, out.size)
+    if r < 0:
+        raise PvvcError(1, "pvvc_enc_colors_on_tree")
+    return out[:r].tobytes()
+
+
 def encode_colors_cpu(xyz: np.ndarray, yuv: np.ndarray, qp: int) -> tuple[bytes, np.ndarray, bytes]:
     """The stream writer's CPU encoder of one patch: (octree bytes, leaf YUV, RLGR bytes)."""
     L = enc_lib()

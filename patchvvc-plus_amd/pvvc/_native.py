@@ -175,6 +175,7 @@ def enc_lib() -> C.CDLL:
     _sig(L, "pvvc_enc_octree", C.c_int64, c_vp, C.c_int, c_vp, C.c_int64, c_i32p)
     _sig(L, "pvvc_enc_raht", C.c_int64, c_vp, C.c_int, c_vp, c_vp)
     _sig(L, "pvvc_enc_rlgr", C.c_int64, c_vp, C.c_int64, c_vp, C.c_int64)
+    _sig(L, "pvvc_enc_colors_on_tree", C.c_int64, c_vp, C.c_int64, c_vp, C.c_int, C.c_int, c_vp, C.c_int64)
     _sig(L, "pvvc_enc_patch_colors", C.c_int64, c_vp, C.c_int, c_vp, C.c_int, c_vp, C.c_int64, c_i64p, c_vp, c_i32p,
          c_vp, C.c_int64)
     _sig(L, "pvvc_enc_gop_patch", C.c_int64, c_vp, C.c_int, c_vp, C.c_int, C.c_int, C.c_int, C.c_int, c_vp,

@@ -74,3 +74,47 @@ def test_gpu_encoder_rejects_bad_geometry(pvvc):
     bad[24] = 0  # height 0
     outs, status, _ = pvvc.encode_colors([(bytes(bad), leaf, qp), (geo, leaf[:-1], qp)])
     assert status == [1, 3]  # BAD_SLICE, UNMATCHED_SIZE (leaf count != n)
+
+
+def test_cpu_encoder_on_tree_equals_point_encoder(pvvc):
+    """The CPU encoder on given octree bytes (the reference for GPU parity at
+    full size) produces the same stream as the writer's encoder from points."""
+    jobs, refs = _jobs(pvvc, seed=11, count=12)
+    for (geo, leaf, qp), col in zip(jobs, refs):
+        assert pvvc.encode_colors_cpu_on_tree(geo, leaf, qp) == col
+
+
+def _boundary_jobs(pvvc):
+    """Streams whose byte length ends on or just short of a 256-byte (64-word)
+    boundary: the writer's last partial word then completes or nearly
+    completes a store chunk.  Found by a seeded search over small random
+    patches; every residue class below is required."""
+    rng = np.random.default_rng(21)
+    want = {253: None, 254: None, 255: None, 0: None, 1: None, 252: None}
+    for _ in range(4000):
+        n = int(rng.integers(40, 700))
+        v = np.unique(rng.integers(0, 24, (n, 3)), axis=0).astype(np.float32)
+        yuv = (128 + rng.normal(0, rng.uniform(2, 30), (len(v), 3))).astype(np.float32)
+        qp = int(rng.choice([1, 2, 4]))
+        geo, leaf, col = pvvc.encode_colors_cpu(v, yuv, qp)
+        r = len(col) % 256
+        if len(col) >= 256 and r in want and want[r] is None:
+            want[r] = ((geo, leaf, qp), col)
+        if all(x is not None for x in want.values()):
+            break
+    assert all(x is not None for x in want.values()), {k: x is not None for k, x in want.items()}
+    return [x[0] for x in want.values()], [x[1] for x in want.values()]
+
+
+def test_boundary_search_finds_every_residue(pvvc):
+    jobs, refs = _boundary_jobs(pvvc)
+    assert sorted(len(r) % 256 for r in refs) == [0, 1, 252, 253, 254, 255]
+
+
+@pytest.mark.gpu
+def test_gpu_encoder_chunk_boundaries(pvvc):
+    jobs, refs = _boundary_jobs(pvvc)
+    outs, status, _ = pvvc.encode_colors(jobs)
+    assert status == [0] * len(jobs), status
+    for o, r in zip(outs, refs):
+        assert o == r, f"{len(r)} bytes (mod 256 = {len(r) % 256})"
