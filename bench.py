@@ -63,8 +63,8 @@ def measure_traffic(args) -> dict | None:
     if not prof:
         return None
     me = [sys.executable, "-u", str(Path(__file__).resolve())]
-    leg = me + ["--only", "batched", "--steps", "2", "--warmup", "1", "--no-cpu", "--no-traffic",
-                "--preset", args.preset, "--seed", str(args.seed)]
+    leg = me + ["--only", "batched", "--steps", "2", "--warmup", "1", "--no-cpu", "--no-traffic", "--no-encode",
+                "--no-quality", "--preset", args.preset, "--seed", str(args.seed)]
     for k in ("waves", "lds", "wgs_per_cu", "queue_chunk", "priority_items"):
         v = getattr(args, k)
         if v:
@@ -167,6 +167,7 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
     ap.add_argument("--no-serving", action="store_true", help="skip the multi-stream serving leg")
     ap.add_argument("--no-encode", action="store_true", help="skip the GPU colour-encoder leg")
+    ap.add_argument("--no-quality", action="store_true", help="skip the MSE / PSNR leg")
     ap.add_argument("--serve-streams", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-lanes", type=int, default=3)
@@ -205,7 +206,8 @@ def main() -> int:
         return 0
     gop = pvvc.SYNTH_PRESETS[args.preset]["gop"]
     t0 = time.time()
-    images = pvvc.synth_stream(args.preset, n_frames=gop * args.gops, seed=args.seed, threads=min(16, os.cpu_count() or 8))
+    images, sources = pvvc.synth_stream(args.preset, n_frames=gop * args.gops, seed=args.seed,
+                                        threads=min(16, os.cpu_count() or 8), keep_source=True)
     log(f"generated {len(images)} frames in {time.time() - t0:.1f}s")
     frames = [pvvc.Frame.parse(b) for b in images]
     metas = [f.meta() for f in frames]
@@ -398,6 +400,34 @@ def main() -> int:
                   "frame0_streams_equal_to_cpu": f"{equal}/{ns0}"}
         log("encode", encode)
 
+    # ---------------- quality: MSE / PSNR of decoded frames vs their source ----------------
+    # MSE::Compute + the PSNR PVVCDecompression prints (geometry peak 1024, Y
+    # peak 256), on the GPU over the batched output of frames 0 and 1; the CPU
+    # oracle's grid search on frame 0 gives the same neighbours and its time.
+    quality = None
+    if run_b and not args.no_quality:
+        xyz_q, rgb_q = dec.decode_host(batches[0], pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+        per = []
+        for f in range(min(2, gop)):
+            o_, n_ = batches[0].frame_offset[f], batches[0].frame_points[f]
+            dcl = (xyz_q[o_:o_ + n_], rgb_q[o_:o_ + n_])
+            pvvc.quality.cloud_mse(sources[f], dcl)  # warm
+            r_ = pvvc.quality.cloud_mse(sources[f], dcl, want_nn=not args.no_cpu and f == 0)
+            row = {"frame": f, "source_points": len(sources[f][0]), "decoded_points": n_,
+                   "geo_psnr_db": round(pvvc.quality.psnr(r_["geo"], pvvc.quality.GEO_PEAK), 3),
+                   "y_psnr_db": round(pvvc.quality.psnr(r_["y"], pvvc.quality.Y_PEAK), 3),
+                   "gpu_ms": round(r_["kernel_ms"], 3)}
+            if not args.no_cpu and f == 0:
+                from oracle import binding
+                t_q = time.perf_counter()
+                oq = binding.cloud_mse(*sources[f], *dcl, threads=min(16, os.cpu_count() or 8), want_nn=True)
+                row["cpu_oracle_ms"] = round((time.perf_counter() - t_q) * 1e3, 1)
+                row["neighbours_equal_to_oracle"] = bool(np.array_equal(oq["nn_p"], r_["nn_p"]) and
+                                                         np.array_equal(oq["nn_q"], r_["nn_q"]))
+            per.append(row)
+        quality = {"frames": per}
+        log("quality", quality)
+
     # ---------------- serving loop: many concurrent streams ----------------
     serving = None
     if not args.no_serving and args.only == "all":
@@ -446,7 +476,7 @@ def main() -> int:
             # colour accuracy of the batched GPU output against the FP64 inverse-RAHT
             # oracle on the first two frames of the GoP (one intra, one predictive)
             if run_b:
-                _, rgb_g = dec.decode_host(batches[0], pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+                xyz_g, rgb_g = dec.decode_host(batches[0], pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
                 o64 = binding.StreamDecoder(fp64=True)
                 diffs, exact, geo_ok, cnt = [], 0, True, 0
                 for k in range(min(2, gop)):
@@ -497,6 +527,7 @@ def main() -> int:
         "rlgr_symbols_per_s_per_cu": round(sym_per_s_cu, 1),
         "end_to_end_per_frame": e2e,
         "encode": encode,
+        "quality": quality,
         "accuracy": accuracy,
         "serving": serving,
         "cpu_baseline": cpu,

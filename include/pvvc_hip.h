@@ -249,6 +249,36 @@ typedef struct pvvc_color_job {
 int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms);
 
 /* ------------------------------------------------------------------------ */
+/* Quality metric: symmetric nearest-neighbour MSE of two coloured clouds,   */
+/* replacing vvc::common::MSE (SetClouds + Compute + Get*MSEs,               */
+/* include/registration/registration.h:48-100, registration_base.cc:159-238). */
+/* ------------------------------------------------------------------------ */
+typedef struct pvvc_mse {
+    /* {geo, y, u, v}: means over the points of p of the squared distance to
+     * the nearest point of q and of the squared Y/U/V difference to it
+     * (MSE::Compute's `.first`); q_to_p the same from q (`.second`).  -1 for
+     * an empty cloud's direction; the other direction is then left at 0. */
+    double p_to_q[4];
+    double q_to_p[4];
+} pvvc_mse;
+/* p_xyz/q_xyz: device f32 x3 per point; p_rgb/q_rgb: device RGB8 x3 (the
+ * decoder's native output layout).  Neighbour = least squared distance in
+ * FP32, (dx*dx + dy*dy) + dz*dz; among equidistant points the lowest index
+ * (the reference's FLANN k-d tree leaves that order unspecified).  YUV of an
+ * RGB8 point as ColorYUV(PointXYZRGB) (include/common/common.h:95-99).
+ * Sums are FP64 in a fixed order, so results are deterministic; the
+ * reference keeps float running sums.  nn_p / nn_q (optional, device int64
+ * per point) receive each point's neighbour index.  Blocks until done;
+ * kernel_ms (optional) receives the device time of the whole computation. */
+int pvvc_cloud_mse(const float* p_xyz, const uint8_t* p_rgb, uint64_t np, const float* q_xyz,
+                   const uint8_t* q_rgb, uint64_t nq, int64_t* nn_p, int64_t* nn_q, pvvc_mse* out,
+                   double* kernel_ms);
+/* PSNR as PVVCDecompression prints it (pvvc_decompression.cc:156-159), in
+ * FP32: 10 log10(peak^2 / max(a, b)); the reference's peaks are 1024 for
+ * geometry and 256 for Y. */
+float pvvc_psnr(double mse_a, double mse_b, float peak);
+
+/* ------------------------------------------------------------------------ */
 /* Serving loop: many concurrent streams, continuous batching.  The          */
 /* reference client decodes one stream with a loader thread and a decoder   */
 /* thread around launch_cudaProcess (src/cuda/common/manager.cc:145-310);    */

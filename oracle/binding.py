@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
     sig("orc_ctx_free", None, vp)
     sig("orc_ctx_set_fp64", None, vp, C.c_int)
     sig("orc_decode_frame", C.c_int64, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp)
+    sig("orc_cloud_mse", C.c_int, vp, vp, C.c_int64, vp, vp, C.c_int64, C.c_int, C.c_int, vp, vp, vp)
     _lib = L
     return L
 
@@ -190,3 +191,28 @@ class StreamDecoder:
         lib().orc_decode_frame(self.h, frame.h, threads, buffers["xyz"].ctypes.data, buffers["rgb"].ctypes.data,
                                None, None, None, buffers["status"].ctypes.data, C.byref(sat), ms.ctypes.data)
         return float(ms[0]), float(ms[1])
+
+
+MSE_BRUTE = 1
+MSE_FLOAT_RUNNING = 2
+
+
+def cloud_mse(p_xyz, p_rgb, q_xyz, q_rgb, mode: int = 0, threads: int = 8, want_nn: bool = False):
+    """MSE::Compute restated (registration_base.cc:168-223).  Returns a dict
+    geo/y/u/v -> (p->q, q->p) and, with want_nn, the neighbour indices."""
+    px = np.ascontiguousarray(p_xyz, np.float32).reshape(-1, 3)
+    qx = np.ascontiguousarray(q_xyz, np.float32).reshape(-1, 3)
+    pc = np.ascontiguousarray(p_rgb, np.uint8).reshape(-1, 3)
+    qc = np.ascontiguousarray(q_rgb, np.uint8).reshape(-1, 3)
+    assert len(px) == len(pc) and len(qx) == len(qc)
+    out = np.zeros(8, np.float64)
+    nn_p = np.zeros(max(1, len(px)), np.int64)
+    nn_q = np.zeros(max(1, len(qx)), np.int64)
+    rc = lib().orc_cloud_mse(px.ctypes.data, pc.ctypes.data, len(px), qx.ctypes.data, qc.ctypes.data, len(qx),
+                             mode, threads, out.ctypes.data, nn_p.ctypes.data, nn_q.ctypes.data)
+    assert rc == 0
+    res = {k: (float(out[i]), float(out[4 + i])) for i, k in enumerate(("geo", "y", "u", "v"))}
+    if want_nn:
+        res["nn_p"] = nn_p[: len(px)]
+        res["nn_q"] = nn_q[: len(qx)]
+    return res

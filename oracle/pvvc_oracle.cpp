@@ -739,6 +739,170 @@ int load_frame(const char* path, Frame& f) {
     return rc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Quality metric: symmetric nearest-neighbour MSE, MSE::Compute            */
+/* (src/registration/registration_base.cc:168-223).                         */
+/*                                                                          */
+/* For every point of p, its nearest point in q (squared Euclidean distance, */
+/* the value PCL's KdTree::nearestKSearch returns), and the squared Y/U/V    */
+/* differences of the pair; then the same from q to p.  Means over each      */
+/* cloud.  Defined where the reference is implementation-defined:           */
+/*   - equidistant candidates: the lowest index wins (FLANN's order is      */
+/*     unspecified);                                                        */
+/*   - sums are FP64 over the exact FP64 squares of the FP32 terms; the      */
+/*     reference keeps a float running sum (mode bit 1 reproduces that one  */
+/*     in query order).                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* squared distance, FP32, (dx*dx + dy*dy) + dz*dz (no FMA) */
+inline float nn_d2(const float* a, const float* b) {
+    const float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+/* ColorYUV(const PointXYZRGB&), include/common/common.h:95-99 */
+inline void nn_yuv(const uint8_t* c, float* o) {
+    const float r = c[0], g = c[1], b = c[2];
+    o[0] = r * 0.299f + g * 0.587f + b * 0.114f;
+    o[1] = r * -0.168736f - g * 0.331264f + b * 0.5f + 128;
+    o[2] = r * 0.5f - g * 0.418688f - b * 0.081312f + 128;
+}
+
+/* Nearest neighbours of every query in `tgt` by brute force. */
+void nn_brute(const float* qry, int64_t nq, const float* tgt, int64_t nt, int64_t* idx, float* d2) {
+    for (int64_t i = 0; i < nq; ++i) {
+        float best = INFINITY;
+        int64_t bi = -1;
+        for (int64_t j = 0; j < nt; ++j) {
+            const float d = nn_d2(qry + 3 * i, tgt + 3 * j);
+            if (d < best) best = d, bi = j;
+        }
+        idx[i] = bi;
+        d2[i] = best;
+    }
+}
+
+/* Nearest neighbours by a uniform grid of cubic cells (hash of cell ->
+ * sorted run) and a search over shells of cells of growing Chebyshev
+ * radius, stopped once no unvisited cell can hold a point at or below the
+ * best distance.  Same answers as nn_brute (lowest index on ties). */
+struct NnGrid {
+    double org[3] = {0, 0, 0}, h = 1;
+    int64_t dim[3] = {1, 1, 1};
+    std::vector<int64_t> order;                 /* target indices sorted by (cell, index) */
+    std::vector<uint64_t> key;                  /* cell key of order[i] */
+    std::vector<std::pair<uint64_t, int64_t>> head; /* (cell key, first position), sorted */
+
+    void cell_of(const float* p, int64_t* c) const {
+        for (int a = 0; a < 3; ++a) {
+            int64_t v = static_cast<int64_t>(std::floor((double(p[a]) - org[a]) / h));
+            c[a] = std::min(std::max<int64_t>(v, 0), dim[a] - 1);
+        }
+    }
+    static uint64_t pack(const int64_t* c) {
+        return (uint64_t(c[0]) << 42) | (uint64_t(c[1]) << 21) | uint64_t(c[2]);
+    }
+    void build(const float* t, int64_t n) {
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], double(t[3 * i + a])), hi[a] = std::max(hi[a], double(t[3 * i + a]));
+        double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-6});
+        h = std::max(2.0 * ext / std::sqrt(double(std::max<int64_t>(n, 1))), ext / 1.0e6);
+        for (int a = 0; a < 3; ++a) {
+            org[a] = lo[a];
+            dim[a] = std::min<int64_t>(int64_t((hi[a] - lo[a]) / h) + 1, (int64_t(1) << 21) - 1);
+        }
+        std::vector<std::pair<uint64_t, int64_t>> kv(n);
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t c[3];
+            cell_of(t + 3 * i, c);
+            kv[i] = {pack(c), i};
+        }
+        std::sort(kv.begin(), kv.end());
+        order.resize(n);
+        key.resize(n);
+        head.clear();
+        for (int64_t i = 0; i < n; ++i) {
+            order[i] = kv[i].second;
+            key[i] = kv[i].first;
+            if (i == 0 || kv[i].first != kv[i - 1].first) head.push_back({kv[i].first, i});
+        }
+    }
+    void query(const float* t, const float* q, int64_t& bi, float& best) const {
+        int64_t c[3];
+        cell_of(q, c);
+        best = INFINITY;
+        bi = -1;
+        const int64_t rmax = std::max({dim[0], dim[1], dim[2]});
+        for (int64_t r = 0; r <= rmax; ++r) {
+            for (int64_t dx = -r; dx <= r; ++dx)
+                for (int64_t dy = -r; dy <= r; ++dy)
+                    for (int64_t dz = -r; dz <= r; ++dz) {
+                        if (std::max({std::llabs(dx), std::llabs(dy), std::llabs(dz)}) != r) continue;
+                        int64_t cc[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
+                        if (cc[0] < 0 || cc[1] < 0 || cc[2] < 0 || cc[0] >= dim[0] || cc[1] >= dim[1] || cc[2] >= dim[2])
+                            continue;
+                        const uint64_t k = pack(cc);
+                        auto it = std::lower_bound(head.begin(), head.end(), std::make_pair(k, int64_t(-1)));
+                        if (it == head.end() || it->first != k) continue;
+                        for (int64_t j = it->second; j < static_cast<int64_t>(key.size()) && key[j] == k; ++j) {
+                            const int64_t ti = order[j];
+                            const float d = nn_d2(q, t + 3 * ti);
+                            if (d < best || (d == best && ti < bi)) best = d, bi = ti;
+                        }
+                    }
+            /* every unvisited point lies outside cells [c-r, c+r]: at least lb away */
+            double lb = INFINITY;
+            for (int a = 0; a < 3; ++a) {
+                const double lo_a = org[a] + double(c[a] - r) * h, hi_a = org[a] + double(c[a] + r + 1) * h;
+                if (c[a] - r > 0) lb = std::min(lb, double(q[a]) - lo_a);
+                if (c[a] + r < dim[a] - 1) lb = std::min(lb, hi_a - double(q[a]));
+            }
+            if (lb == INFINITY) break; /* the cube covers the grid */
+            lb -= 1e-9 * (std::fabs(double(q[0])) + std::fabs(double(q[1])) + std::fabs(double(q[2])) + h);
+            if (lb > 0 && double(best) < lb * lb * (1.0 - 1e-6)) break;
+        }
+    }
+};
+
+void nn_grid(const float* qry, int64_t nq, const float* tgt, int64_t nt, int64_t* idx, float* d2, int threads) {
+    NnGrid g;
+    g.build(tgt, nt);
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int64_t b = next.fetch_add(4096);
+            if (b >= nq) break;
+            for (int64_t i = b; i < std::min(nq, b + 4096); ++i) g.query(tgt, qry + 3 * i, idx[i], d2[i]);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < std::max(1, threads); ++i) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
+}
+
+/* One direction of MSE::Compute: sums over queries (in query order) of the
+ * squared distance and the squared Y, U, V differences to the neighbour. */
+void mse_sums(const float* qx, const uint8_t* qc, int64_t nq, const uint8_t* tc, const int64_t* idx,
+              const float* d2, bool float_running, double* out4) {
+    double s[4] = {0, 0, 0, 0};
+    float f[4] = {0, 0, 0, 0};
+    for (int64_t i = 0; i < nq; ++i) {
+        float a[3], b[3];
+        nn_yuv(qc + 3 * i, a);
+        nn_yuv(tc + 3 * idx[i], b);
+        double t[4] = {double(d2[i]), std::pow(double(a[0] - b[0]), 2), std::pow(double(a[1] - b[1]), 2),
+                       std::pow(double(a[2] - b[2]), 2)};
+        for (int k = 0; k < 4; ++k) {
+            s[k] += t[k];
+            f[k] = static_cast<float>(double(f[k]) + t[k]); /* float += double */
+        }
+    }
+    (void)qx;
+    for (int k = 0; k < 4; ++k) out4[k] = float_running ? double(f[k] / float(nq)) : s[k] / double(nq);
+}
+
 }  // namespace
 
 /* ======================================================================== */
@@ -943,6 +1107,35 @@ int64_t orc_decode_frame(void* hctx, void* hframe, int nthreads, float* xyz, uin
         timed_ms[1] = std::chrono::duration<double, std::milli>(t2 - t0).count();
     }
     return off[S];
+}
+
+/* MSE::Compute (registration_base.cc:168-223).  out8 = {geo, y, u, v} of
+ * p->q then {geo, y, u, v} of q->p.  mode bit 0: brute-force neighbours
+ * (else the grid); bit 1: the reference's float running sums.  nn_p/nn_q
+ * (optional) receive the neighbour index of each point.  An empty cloud
+ * gives -1 for its direction and leaves the other at 0 (:170-179). */
+int orc_cloud_mse(const float* pxyz, const uint8_t* prgb, int64_t np, const float* qxyz, const uint8_t* qrgb,
+                  int64_t nq, int mode, int threads, double* out8, int64_t* nn_p, int64_t* nn_q) {
+    for (int k = 0; k < 8; ++k) out8[k] = 0.0;
+    if (np <= 0)
+        for (int k = 0; k < 4; ++k) out8[k] = -1.0;
+    if (nq <= 0)
+        for (int k = 4; k < 8; ++k) out8[k] = -1.0;
+    if (np <= 0 || nq <= 0) return 0;
+    std::vector<int64_t> ip(np), iq(nq);
+    std::vector<float> dp(np), dq(nq);
+    if (mode & 1) {
+        nn_brute(pxyz, np, qxyz, nq, ip.data(), dp.data());
+        nn_brute(qxyz, nq, pxyz, np, iq.data(), dq.data());
+    } else {
+        nn_grid(pxyz, np, qxyz, nq, ip.data(), dp.data(), threads);
+        nn_grid(qxyz, nq, pxyz, np, iq.data(), dq.data(), threads);
+    }
+    mse_sums(pxyz, prgb, np, qrgb, ip.data(), dp.data(), (mode & 2) != 0, out8);
+    mse_sums(qxyz, qrgb, nq, prgb, iq.data(), dq.data(), (mode & 2) != 0, out8 + 4);
+    if (nn_p) std::memcpy(nn_p, ip.data(), sizeof(int64_t) * np);
+    if (nn_q) std::memcpy(nn_q, iq.data(), sizeof(int64_t) * nq);
+    return 0;
 }
 
 }  // extern "C"

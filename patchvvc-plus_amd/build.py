@@ -67,6 +67,8 @@ def build(jobs: int = 4) -> list[Path]:
                                 str(obj / "pvvc_decode.o")],
         obj / "pvvc_serve.o": [HIPCC, *HIP_FLAGS, f"-I{inc}", "-c", str(CSRC / "pvvc_serve.hip"), "-o",
                                str(obj / "pvvc_serve.o")],
+        obj / "pvvc_metrics.o": [HIPCC, *HIP_FLAGS, f"-I{inc}", "-c", str(CSRC / "pvvc_metrics.hip"), "-o",
+                                 str(obj / "pvvc_metrics.o")],
         OUT / "libpvvc_enc.so": [CXX, *CXX_FLAGS, f"-I{inc}", "-shared", str(CSRC / "stream_writer.cpp"), "-o",
                                  str(OUT / "libpvvc_enc.so"), zlib, "-lpthread"],
     }
@@ -74,7 +76,7 @@ def build(jobs: int = 4) -> list[Path]:
         list(ex.map(run, steps.values()))
     hip_so = OUT / "libpvvc_hip.so"
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(obj / "pvvc_decode.o"), str(obj / "pvvc_serve.o"),
-         str(obj / "frame_io.o"),
+         str(obj / "pvvc_metrics.o"), str(obj / "frame_io.o"),
          "-o", str(hip_so), zlib, "-lpthread"])
     shutil.rmtree(obj, ignore_errors=True)
     return [hip_so, OUT / "libpvvc_enc.so"]

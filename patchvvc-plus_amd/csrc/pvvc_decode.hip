@@ -1963,6 +1963,7 @@ void rt_timing_reset() {
     g_ktiming.reset();
 }
 void rt_set_error(int e) { set_error(e); }
+hipStream_t rt_stream() { return g_stream; }
 
 void rt_init_kernels() {
     static std::once_flag once;

@@ -73,6 +73,8 @@ void launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, cons
 /* Records a HIP failure in the library's sticky error; true on success. */
 bool rt_hip_ok(hipError_t e);
 void rt_set_error(int e);
+/* The library stream (pvvc_set_stream; null stream by default). */
+hipStream_t rt_stream();
 /* Sets the dynamic-LDS limits of the decode kernels (once per process). */
 void rt_init_kernels();
 /* Forgets the calling thread's launch-time records (their events are reused). */

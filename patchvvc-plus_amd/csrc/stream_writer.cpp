@@ -536,6 +536,7 @@ struct SynthCfg {
     int grid_bits, n_frames, gop, qp_i, qp_p, n_humanoids, jitter, patch_min, patch_max, zstd_level, threads;
     double scale, shell, patch_mu, patch_sigma, noise_sigma, max_rot_deg, max_trans;
     uint64_t seed;
+    int keep_source; /* keep each frame's source cloud (world xyz + RGB8) */
 };
 
 /* Pose of primitive p (global index) at frame t: a seeded rotation of at
@@ -662,12 +663,17 @@ typedef struct pvvc_synth_config {
     int grid_bits, n_frames, gop, qp_i, qp_p, n_humanoids, jitter, patch_min, patch_max, zstd_level, threads;
     double scale, shell, patch_mu, patch_sigma, noise_sigma, max_rot_deg, max_trans;
     uint64_t seed;
+    int keep_source;
 } pvvc_synth_config;
 
 struct pvvc_stream {
     std::vector<std::vector<uint8_t>> frames;
     std::vector<uint64_t> points;
     std::vector<int> patches;
+    /* source clouds (keep_source): per frame, world xyz f32 x3 and RGB8 x3 in
+     * patch order, the cloud the frame was encoded from */
+    std::vector<std::vector<float>> src_xyz;
+    std::vector<std::vector<uint8_t>> src_rgb;
 };
 
 /* Generates n_frames frames; returns 0 or a pvvc_status. */
@@ -697,6 +703,10 @@ int pvvc_synth_generate(const pvvc_synth_config* cfg_in, pvvc_stream** out) {
     st->frames.resize(c.n_frames);
     st->points.resize(c.n_frames, 0);
     st->patches.resize(c.n_frames, 0);
+    if (c.keep_source) {
+        st->src_xyz.resize(c.n_frames);
+        st->src_rgb.resize(c.n_frames);
+    }
     int nt = std::max(1, c.threads);
     for (int t0 = 0; t0 < c.n_frames; t0 += c.gop) {
         const int F = std::min(c.gop, c.n_frames - t0);
@@ -717,6 +727,13 @@ int pvvc_synth_generate(const pvvc_synth_config* cfg_in, pvvc_stream** out) {
         }
         const int S = static_cast<int>(patches.size());
         std::vector<std::vector<Slice>> coded(S, std::vector<Slice>(F));
+        std::vector<size_t> src_off(S + 1, 0);
+        for (int s = 0; s < S; ++s) src_off[s + 1] = src_off[s] + (patches[s].second - patches[s].first);
+        if (c.keep_source)
+            for (int f = 0; f < F; ++f) {
+                st->src_xyz[t0 + f].resize(3 * src_off[S]);
+                st->src_rgb[t0 + f].resize(3 * src_off[S]);
+            }
         std::atomic<int> next{0};
         auto work = [&]() {
             std::vector<XYZ> pts;
@@ -753,6 +770,13 @@ int pvvc_synth_generate(const pvvc_synth_config* cfg_in, pvvc_stream** out) {
                             double v = rgb[k] + c.noise_sigma * gauss(hash3(c.seed ^ 0x1234, (uint64_t(t) << 24) ^ uint64_t(lo + i), k));
                             v = std::min(255.0, std::max(0.0, std::round(v)));
                             ch[k] = static_cast<float>(v);
+                        }
+                        if (c.keep_source) {
+                            const size_t o = 3 * (src_off[s] + i);
+                            for (int k = 0; k < 3; ++k) {
+                                st->src_xyz[t][o + k] = static_cast<float>(w[k]);
+                                st->src_rgb[t][o + k] = static_cast<uint8_t>(ch[k]);
+                            }
                         }
                         YUV y = rgb_to_yuv(ch[0], ch[1], ch[2]);
                         if (f == 0) {
@@ -810,6 +834,15 @@ uint64_t pvvc_stream_frame_points(const pvvc_stream* s, int i) {
     return (s && i >= 0 && i < static_cast<int>(s->points.size())) ? s->points[i] : 0;
 }
 void pvvc_stream_free(pvvc_stream* s) { delete s; }
+
+/* Source cloud of frame i (generated with keep_source): returns the point
+ * count and sets *xyz / *rgb, or 0 when none was kept. */
+uint64_t pvvc_stream_frame_source(const pvvc_stream* s, int i, const float** xyz, const uint8_t** rgb) {
+    if (!s || i < 0 || i >= static_cast<int>(s->src_xyz.size()) || s->src_xyz[i].empty()) return 0;
+    if (xyz) *xyz = s->src_xyz[i].data();
+    if (rgb) *rgb = s->src_rgb[i].data();
+    return s->src_xyz[i].size() / 3;
+}
 
 /* Writes frame i to `path` (the reference names them <seq>_<i>.frame). */
 int pvvc_stream_write_frame(const pvvc_stream* s, int i, const char* path) {

@@ -8,6 +8,7 @@ include/pvvc_hip.h; this module wraps that ABI with ctypes and numpy:
                         (include/cuda/render.cuh:41-57) with its device arrays
 * Decoder, Batch    -- native batched decode (per-frame, persistent, graph)
 * synth_stream      -- seeded synthetic humanoid streams (stream writer)
+* quality           -- MSE / PSNR of decoded clouds (vvc::common::MSE)
 """
 from __future__ import annotations
 
@@ -626,18 +627,32 @@ def synth_config(preset: str = "c0", n_frames: int | None = None, seed: int = 1,
 
 
 def synth_stream(preset: str = "c0", n_frames: int | None = None, seed: int = 1, threads: int = 8,
-                 zstd_level: int = 3, **over) -> list[bytes]:
-    """Generates a seeded synthetic stream; returns the .frame images."""
+                 zstd_level: int = 3, keep_source: bool = False, **over):
+    """Generates a seeded synthetic stream; returns the .frame images, or,
+    with keep_source, (images, sources) where sources[i] = (xyz f32 [n,3],
+    rgb u8 [n,3]) is the cloud frame i was encoded from."""
     L = enc_lib()
     cfg = synth_config(preset, n_frames, seed, threads, zstd_level, **over)
+    cfg.keep_source = int(bool(keep_source))
     h = C.c_void_p()
     _check(L.pvvc_synth_generate(C.byref(cfg), C.byref(h)), "pvvc_synth_generate")
     try:
-        out = []
+        out, src = [], []
         for i in range(L.pvvc_stream_frames(h)):
             n = C.c_size_t()
             p = L.pvvc_stream_frame_bytes(h, i, C.byref(n))
             out.append(C.string_at(p, n.value))
-        return out
+            if keep_source:
+                xp, rp = C.c_void_p(), C.c_void_p()
+                m = L.pvvc_stream_frame_source(h, i, C.byref(xp), C.byref(rp))
+                xyz = np.ctypeslib.as_array(C.cast(xp, C.POINTER(C.c_float)), (m, 3)).copy() if m else \
+                    np.zeros((0, 3), np.float32)
+                rgb = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_uint8)), (m, 3)).copy() if m else \
+                    np.zeros((0, 3), np.uint8)
+                src.append((xyz, rgb))
+        return (out, src) if keep_source else out
     finally:
         L.pvvc_stream_free(h)
+
+
+from . import quality  # noqa: E402,F401  (MSE / PSNR; imports DeviceBuffer lazily)

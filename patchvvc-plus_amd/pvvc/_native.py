@@ -62,11 +62,15 @@ class ColorJob(C.Structure):
                 ("status", C.c_int)]
 
 
+class Mse(C.Structure):
+    _fields_ = [("p_to_q", C.c_double * 4), ("q_to_p", C.c_double * 4)]
+
+
 class SynthConfig(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("grid_bits", "n_frames", "gop", "qp_i", "qp_p", "n_humanoids", "jitter",
                                        "patch_min", "patch_max", "zstd_level", "threads")] + \
                [(n, C.c_double) for n in ("scale", "shell", "patch_mu", "patch_sigma", "noise_sigma", "max_rot_deg",
-                                          "max_trans")] + [("seed", C.c_uint64)]
+                                          "max_trans")] + [("seed", C.c_uint64), ("keep_source", C.c_int)]
 
 
 def _sig(lib, name, restype, *argtypes):
@@ -131,6 +135,9 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_batch_status", C.c_int, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_decode_batch", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp)
     _sig(L, "pvvc_encode_colors", C.c_int, C.POINTER(ColorJob), C.c_int, C.POINTER(C.c_double))
+    _sig(L, "pvvc_cloud_mse", C.c_int, c_vp, c_vp, C.c_uint64, c_vp, c_vp, C.c_uint64, c_vp, c_vp, C.POINTER(Mse),
+         C.POINTER(C.c_double))
+    _sig(L, "pvvc_psnr", C.c_float, C.c_double, C.c_double, C.c_float)
     _sig(L, "pvvc_server_clock", C.c_double)
     _sig(L, "pvvc_server_create", C.c_int, C.POINTER(ServerConfig), C.POINTER(c_vp))
     _sig(L, "pvvc_server_destroy", C.c_int, c_vp)
@@ -170,6 +177,7 @@ def enc_lib() -> C.CDLL:
     _sig(L, "pvvc_stream_frames", C.c_int, c_vp)
     _sig(L, "pvvc_stream_frame_bytes", c_vp, c_vp, C.c_int, C.POINTER(C.c_size_t))
     _sig(L, "pvvc_stream_frame_points", C.c_uint64, c_vp, C.c_int)
+    _sig(L, "pvvc_stream_frame_source", C.c_uint64, c_vp, C.c_int, C.POINTER(c_vp), C.POINTER(c_vp))
     _sig(L, "pvvc_stream_write_frame", C.c_int, c_vp, C.c_int, C.c_char_p)
     _sig(L, "pvvc_stream_free", None, c_vp)
     _sig(L, "pvvc_enc_octree", C.c_int64, c_vp, C.c_int, c_vp, C.c_int64, c_i32p)
