@@ -32,8 +32,10 @@ def c0_oracle(oracle, c0_files):
     return _oracle_decode(oracle, c0_files)
 
 
-def test_dropin_c0_matches_oracle(pvvc, c0_files, c0_oracle):
-    _say("dropin c0: frames", len(c0_files))
+@pytest.mark.parametrize("waves", [2, 1, 4])
+def test_dropin_c0_matches_oracle(pvvc, c0_files, c0_oracle, waves):
+    _say("dropin c0: frames", len(c0_files), "waves", waves)
+    assert pvvc.hip_lib().pvvc_set_dropin_tuning(waves, 32 * 1024) == 0
     frames = [pvvc.Frame.load(str(p)) for p in c0_files]
     n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
     max_pts = max(int(f.meta()["size"].max()) for f in frames)
@@ -48,6 +50,7 @@ def test_dropin_c0_matches_oracle(pvvc, c0_files, c0_oracle):
         np.testing.assert_array_equal(xyz, ref["xyz"])
         rgbf = np.stack([pts["r"], pts["g"], pts["b"]], 1)
         np.testing.assert_array_equal(rgbf, ref["rgbf"] / np.float32(255))
+    assert pvvc.hip_lib().pvvc_set_dropin_tuning(2, 32 * 1024) == 0  # the default
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2], ids=["per_frame", "persistent", "graph"])
