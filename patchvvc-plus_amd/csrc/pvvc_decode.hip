@@ -148,8 +148,14 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * The k = 0 and run cases are two loops (labels 10 and 30), each unrolled
  * twice and handing over to the other when k_P crosses 3, so that the common
  * path falls through: a taken scalar branch costs ~30 cycles on one
- * wavefront, an instruction ~4.  Refills (11/13, 33/43) and the 1-bit run
- * prefixes (31/41) are out of line.
+ * wavefront, an instruction ~4, and the kernel is bound by scalar issue, not
+ * by one wavefront's latency.  A non-zero k = 0 symbol lowers k_P, so it
+ * stays in the k = 0 loop without a mode test; a zero symbol (u = 0) leaves
+ * the line (15/17) to raise k_P by 3, lower k_RP and test the mode, and its
+ * accumulator lane already holds 0.  k_RP's step (p ? p - 1 : -2, clamped to
+ * [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the carry-in of
+ * s_addc from the SCC of p << kR.  Refills (11/13, 18/20, 33/43) and the
+ * 1-bit run prefixes (31/41) are out of line.
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window, so every check precedes any state change.  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
@@ -180,15 +186,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
         "s_lshl_b64 %[win], %[win], s87\n\t"
         "s_lshl_b32 s89, s84, s85\n\t"
-        "s_cselect_b32 s90, -1, -2\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
-        "s_cselect_b32 s88, -1, 3\n\t"
-        "s_add_i32 %[kP], %[kP], s88\n\t"
+        "s_cbranch_scc0 15f\n\t"
+        "s_sub_i32 s90, s90, 2\n\t"
+        "s_max_i32 s90, s90, 0\n\t"
+        "s_min_i32 %[kRP], s90, 128\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
-        "s_add_i32 s88, %[kRP], s84\n\t"
-        "s_add_i32 s88, s88, s90\n\t"
-        "s_max_i32 s88, s88, 0\n\t"
-        "s_min_i32 %[kRP], s88, 128\n\t"
         "v_mov_b32 %[vt], s89\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
         "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
@@ -197,8 +202,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "12:\n\t"
         "s_add_u32 %[neg], %[neg], 1\n\t"
         "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc1 30f\n\t"
+        "16:\n\t"
         /* ---- k = 0 symbol ---- */
         "s_not_b64 s[80:81], %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
@@ -213,15 +217,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
         "s_lshl_b64 %[win], %[win], s87\n\t"
         "s_lshl_b32 s89, s84, s85\n\t"
-        "s_cselect_b32 s90, -1, -2\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
-        "s_cselect_b32 s88, -1, 3\n\t"
-        "s_add_i32 %[kP], %[kP], s88\n\t"
+        "s_cbranch_scc0 17f\n\t"
+        "s_sub_i32 s90, s90, 2\n\t"
+        "s_max_i32 s90, s90, 0\n\t"
+        "s_min_i32 %[kRP], s90, 128\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
-        "s_add_i32 s88, %[kRP], s84\n\t"
-        "s_add_i32 s88, s88, s90\n\t"
-        "s_max_i32 s88, s88, 0\n\t"
-        "s_min_i32 %[kRP], s88, 128\n\t"
         "v_mov_b32 %[vt], s89\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
         "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
@@ -230,8 +233,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "14:\n\t"
         "s_add_u32 %[neg], %[neg], 1\n\t"
         "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc1 30f\n\t"
         "s_branch 10b\n\t"
         "11:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
@@ -259,6 +260,56 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 14b\n\t"
+        "15:\n\t"
+        "s_sub_i32 s90, s90, 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "s_add_u32 %[kP], %[kP], 3\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 18f\n\t"
+        "19:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc1 30f\n\t"
+        "s_branch 16b\n\t"
+        "18:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 19b\n\t"
+        "17:\n\t"
+        "s_sub_i32 s90, s90, 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "s_add_u32 %[kP], %[kP], 3\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 20f\n\t"
+        "21:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc1 30f\n\t"
+        "s_branch 10b\n\t"
+        "20:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 21b\n\t"
         "30:\n\t"
         /* ---- k > 0: run entry ---- */
         "s_not_b64 s[80:81], %[win]\n\t"
@@ -291,14 +342,13 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
         "s_lshl_b64 %[win], %[win], s86\n\t"
         "s_lshl_b32 s88, s84, s89\n\t"
-        "s_cselect_b32 s91, -1, -2\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
         "s_sub_u32 %[kP], s87, 1\n\t"
-        "s_add_i32 s90, %[kRP], s84\n\t"
-        "s_add_i32 s90, s90, s91\n\t"
-        "s_max_i32 s90, s90, 0\n\t"
-        "s_min_i32 %[kRP], s90, 128\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 s91, s91, 0\n\t"
+        "s_min_i32 %[kRP], s91, 128\n\t"
         "s_mov_b32 %[neg], s83\n\t"
         "v_mov_b32 %[vt], s88\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
@@ -341,14 +391,13 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
         "s_lshl_b64 %[win], %[win], s86\n\t"
         "s_lshl_b32 s88, s84, s89\n\t"
-        "s_cselect_b32 s91, -1, -2\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
         "s_sub_u32 %[kP], s87, 1\n\t"
-        "s_add_i32 s90, %[kRP], s84\n\t"
-        "s_add_i32 s90, s90, s91\n\t"
-        "s_max_i32 s90, s90, 0\n\t"
-        "s_min_i32 %[kRP], s90, 128\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 s91, s91, 0\n\t"
+        "s_min_i32 %[kRP], s91, 128\n\t"
         "s_mov_b32 %[neg], s83\n\t"
         "v_mov_b32 %[vt], s88\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
