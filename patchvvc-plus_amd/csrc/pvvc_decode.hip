@@ -154,8 +154,11 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * the line (15/17) to raise k_P by 3, lower k_RP and test the mode, and its
  * accumulator lane already holds 0.  k_RP's step (p ? p - 1 : -2, clamped to
  * [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the carry-in of
- * s_addc from the SCC of p << kR.  Refills (11/13, 18/20, 33/43) and the
- * 1-bit run prefixes (31/41) are out of line.
+ * s_addc from the SCC of p << kR.  A run entry whose prefix has no one-bits
+ * (the common case) has its own path (30/40): its zero count is the k-bit
+ * suffix alone and `neg` advances in place (undone on the out-of-line exits
+ * 37/47); prefixes with one-bits take the general path (36/46, with the
+ * loop 31/41).  Refills (11/13, 18/20, 33/43, 38/48) are out of line.
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window, so every check precedes any state change.  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
@@ -311,32 +314,26 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 21b\n\t"
         "30:\n\t"
-        /* ---- k > 0: run entry ---- */
+        /* ---- k > 0: run entry, no one-bits ---- */
         "s_not_b64 s[80:81], %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_mov_b32 s82, 0\n\t"
-        "s_mov_b32 s87, %[kP]\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
-        "s_cbranch_scc1 31f\n\t"
-        "s_lshr_b32 s85, s87, 2\n\t"
-        "32:\n\t"
-        "s_add_u32 s86, s84, 1\n\t"
-        "s_lshl_b64 s[80:81], %[win], s86\n\t"
+        "s_cbranch_scc1 36f\n\t"
+        "s_lshr_b32 s85, %[kP], 2\n\t"
+        "s_lshl_b64 s[80:81], %[win], 1\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_add_u32 s82, s82, s90\n\t"
-        "s_add_u32 s83, %[neg], s82\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_add_u32 %[neg], %[neg], s90\n\t"
+        "s_cbranch_scc1 37f\n\t"
         "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
-        "s_add_u32 s86, s86, s85\n\t"
         "s_not_b64 s[80:81], s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_add_u32 s86, s85, 2\n\t"
         "s_add_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_add_u32 s86, s86, 1\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_cbranch_scc1 37f\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
@@ -345,11 +342,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
-        "s_sub_u32 %[kP], s87, 1\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 s91, s91, 0\n\t"
         "s_min_i32 %[kRP], s91, 128\n\t"
-        "s_mov_b32 %[neg], s83\n\t"
         "v_mov_b32 %[vt], s88\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
         "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
@@ -360,32 +356,28 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
-        /* ---- k > 0: run entry ---- */
+        "s_branch 40f\n\t"
+        "40:\n\t"
+        /* ---- k > 0: run entry, no one-bits ---- */
         "s_not_b64 s[80:81], %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_mov_b32 s82, 0\n\t"
-        "s_mov_b32 s87, %[kP]\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
-        "s_cbranch_scc1 41f\n\t"
-        "s_lshr_b32 s85, s87, 2\n\t"
-        "42:\n\t"
-        "s_add_u32 s86, s84, 1\n\t"
-        "s_lshl_b64 s[80:81], %[win], s86\n\t"
+        "s_cbranch_scc1 46f\n\t"
+        "s_lshr_b32 s85, %[kP], 2\n\t"
+        "s_lshl_b64 s[80:81], %[win], 1\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_add_u32 s82, s82, s90\n\t"
-        "s_add_u32 s83, %[neg], s82\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_add_u32 %[neg], %[neg], s90\n\t"
+        "s_cbranch_scc1 47f\n\t"
         "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
-        "s_add_u32 s86, s86, s85\n\t"
         "s_not_b64 s[80:81], s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_add_u32 s86, s85, 2\n\t"
         "s_add_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_add_u32 s86, s86, 1\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_cbranch_scc1 47f\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
@@ -394,11 +386,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
-        "s_sub_u32 %[kP], s87, 1\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 s91, s91, 0\n\t"
         "s_min_i32 %[kRP], s91, 128\n\t"
-        "s_mov_b32 %[neg], s83\n\t"
         "v_mov_b32 %[vt], s88\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
         "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
@@ -436,6 +427,130 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 44b\n\t"
+        "37:\n\t"
+        "s_sub_u32 %[neg], %[neg], s90\n\t"
+        "s_branch 9f\n\t"
+        "47:\n\t"
+        "s_sub_u32 %[neg], %[neg], s90\n\t"
+        "s_branch 9f\n\t"
+        "36:\n\t"
+        "s_mov_b32 s82, 0\n\t"
+        "s_mov_b32 s87, %[kP]\n\t"
+        "s_branch 31f\n\t"
+        "32:\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
+        "s_lshl_b64 s[80:81], %[win], s86\n\t"
+        "s_sub_u32 s88, 64, s85\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_add_u32 s82, s82, s90\n\t"
+        "s_add_u32 s83, %[neg], s82\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
+        "s_add_u32 s86, s86, s85\n\t"
+        "s_not_b64 s[80:81], s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_add_u32 s86, s86, s84\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_add_u32 s86, s86, 1\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_u32 %[kP], s87, 1\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 s91, s91, 0\n\t"
+        "s_min_i32 %[kRP], s91, 128\n\t"
+        "s_mov_b32 %[neg], s83\n\t"
+        "v_mov_b32 %[vt], s88\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 38f\n\t"
+        "39:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc0 10b\n\t"
+        "s_branch 30b\n\t"
+        "38:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 39b\n\t"
+        "46:\n\t"
+        "s_mov_b32 s82, 0\n\t"
+        "s_mov_b32 s87, %[kP]\n\t"
+        "s_branch 41f\n\t"
+        "42:\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
+        "s_lshl_b64 s[80:81], %[win], s86\n\t"
+        "s_sub_u32 s88, 64, s85\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_add_u32 s82, s82, s90\n\t"
+        "s_add_u32 s83, %[neg], s82\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
+        "s_add_u32 s86, s86, s85\n\t"
+        "s_not_b64 s[80:81], s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_add_u32 s86, s86, s84\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_add_u32 s86, s86, 1\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_u32 %[kP], s87, 1\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 s91, s91, 0\n\t"
+        "s_min_i32 %[kRP], s91, 128\n\t"
+        "s_mov_b32 %[neg], s83\n\t"
+        "v_mov_b32 %[vt], s88\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 48f\n\t"
+        "49:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc0 10b\n\t"
+        "s_branch 30b\n\t"
+        "48:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 49b\n\t"
         "31:\n\t"
         "s_cmp_gt_u32 s84, 6\n\t"
         "s_cbranch_scc1 9f\n\t"
