@@ -14,7 +14,9 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <new>
 #include <numeric>
 #include <tuple>
 #include <vector>
@@ -2726,8 +2728,14 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     const uint64_t total = at;
     uint8_t* dev = nullptr;
     if (!hip_ok(hipMalloc(&dev, total))) return PVVC_ERR_HIP;
-    /* [jobs][results][offsets][inputs]: one host image, one copy */
-    std::vector<uint8_t> h(in_end, 0);
+    /* [jobs][results][offsets][inputs]: one host image, one copy; every byte
+     * the kernel reads is written below (alignment gaps are never read) */
+    std::unique_ptr<uint8_t[]> hbuf(new (std::nothrow) uint8_t[in_end]);
+    if (!hbuf) {
+        hipFree(dev);
+        return PVVC_ERR_CAPACITY;
+    }
+    uint8_t* const h = hbuf.get();
     std::vector<EncJobDev> jd(n_jobs);
     for (int i = 0; i < n_jobs; ++i) {
         EncJobDev& d = jd[i];
@@ -2741,12 +2749,12 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
         d.out_cap = static_cast<uint32_t>(out_cap[i]);
         d.qp = jobs[i].qp;
     }
-    std::memcpy(h.data() + o_jobs, jd.data(), sizeof(EncJobDev) * n_jobs);
+    std::memcpy(h + o_jobs, jd.data(), sizeof(EncJobDev) * n_jobs);
     for (int i = 0; i < n_jobs; ++i) {
-        std::memcpy(h.data() + o_geo[i], jobs[i].geometry, jobs[i].geometry_size);
-        std::memcpy(h.data() + o_yuv[i], jobs[i].yuv, 12ull * jobs[i].n);
+        std::memcpy(h + o_geo[i], jobs[i].geometry, jobs[i].geometry_size);
+        std::memcpy(h + o_yuv[i], jobs[i].yuv, 12ull * jobs[i].n);
     }
-    bool good = hip_ok(hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice));
+    bool good = hip_ok(hipMemcpy(dev, h, in_end, hipMemcpyHostToDevice));
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(16 * 1024));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEventCreate(&e0);

@@ -427,24 +427,48 @@ class Decoder:
 # ---------------------------------------------------------------------------
 # Forward path (colour encoder)
 # ---------------------------------------------------------------------------
+def _color_job_dtype() -> np.dtype:
+    """numpy view of pvvc_color_job (the ctypes ColorJob layout)."""
+    from ._native import ColorJob
+    names = [f for f, _ in ColorJob._fields_]
+    fmt = {"geometry": "<u8", "geometry_size": "<u4", "yuv": "<u8", "n": "<u4", "qp": "u1", "out": "<u8",
+           "out_cap": "<u4", "out_size": "<u4", "status": "<i4"}
+    return np.dtype({"names": names, "formats": [fmt[n] for n in names],
+                     "offsets": [getattr(ColorJob, n).offset for n in names], "itemsize": C.sizeof(ColorJob)})
+
+
 def encode_colors(jobs: Sequence[tuple[bytes, np.ndarray, int]]) -> tuple[list[bytes], list[int], float]:
     """GPU colour encoder: jobs of (octree bytes, leaf YUV float32 [n, 3], qp).
-    Returns (RLGR byte strings, statuses, kernel ms)."""
-    from ._native import ColorJob
+    Returns (RLGR byte strings, statuses, kernel ms).  The job table is
+    filled in one vectorised pass over three concatenated buffers."""
     n = len(jobs)
-    arr = (ColorJob * n)()
-    keep = []
-    for i, (geo, yuv, qp) in enumerate(jobs):
-        g = np.frombuffer(bytes(geo), np.uint8).copy()
-        y = np.ascontiguousarray(yuv, np.float32).reshape(-1, 3)
-        cap = 24 * len(y) + 64
-        o = np.zeros(cap, np.uint8)
-        keep += [g, y, o]
-        arr[i] = ColorJob(g.ctypes.data, g.size, y.ctypes.data, len(y), int(qp), o.ctypes.data, cap, 0, 0)
+    if n == 0:
+        _check(hip_lib().pvvc_encode_colors(None, 0, None), "pvvc_encode_colors")
+    geos = [bytes(j[0]) for j in jobs]
+    yuvs = [np.ascontiguousarray(j[1], np.float32).reshape(-1, 3) for j in jobs]
+    gsz = np.fromiter((len(g) for g in geos), np.int64, n)
+    npts = np.fromiter((len(y) for y in yuvs), np.int64, n)
+    goff = np.concatenate([[0], np.cumsum(gsz)[:-1]]).astype(np.int64)
+    poff = np.concatenate([[0], np.cumsum(npts)[:-1]]).astype(np.int64)
+    cap = 24 * npts + 64
+    ooff = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
+    geo_all = np.frombuffer(b"".join(geos), np.uint8)
+    yuv_all = np.concatenate(yuvs) if n else np.zeros((0, 3), np.float32)
+    out_all = np.zeros(int(cap.sum()) if n else 1, np.uint8)
+    table = np.zeros(n, _color_job_dtype())
+    table["geometry"] = geo_all.ctypes.data + goff
+    table["geometry_size"] = gsz
+    table["yuv"] = yuv_all.ctypes.data + 12 * poff
+    table["n"] = npts
+    table["qp"] = np.fromiter((int(j[2]) for j in jobs), np.uint8, n)
+    table["out"] = out_all.ctypes.data + ooff
+    table["out_cap"] = cap
     ms = C.c_double()
-    _check(hip_lib().pvvc_encode_colors(arr, n, C.byref(ms)), "pvvc_encode_colors")
-    outs = [keep[3 * i + 2][: arr[i].out_size].tobytes() for i in range(n)]
-    return outs, [arr[i].status for i in range(n)], float(ms.value)
+    _check(hip_lib().pvvc_encode_colors(C.cast(table.ctypes.data, C.c_void_p), n, C.byref(ms)),
+           "pvvc_encode_colors")
+    sizes = table["out_size"]
+    outs = [out_all[o:o + z].tobytes() for o, z in zip(ooff.tolist(), sizes.tolist())]
+    return outs, table["status"].tolist(), float(ms.value)
 
 
 def encode_colors_cpu_on_tree(geo: bytes, leaf_yuv: np.ndarray, qp: int) -> bytes:
