@@ -15,7 +15,9 @@
  * centres (header (1,1,1)/(2,2,2)/h=2, occupancy FF and 81, h=1) and by a
  * brute-force recursive expansion; the inverse RAHT by the two-leaf closed
  * form, the constant-colour DC identity, and forward->inverse identity; the
- * container by hand-packed byte fixtures.  The reference itself holds no
+ * patch motion by exact closed forms of non-symmetric integer / power-of-two
+ * matrices; the container by hand-packed byte fixtures; the quality metric
+ * (MSE::Compute) by brute-force neighbours and closed forms.  The reference itself holds no
  * golden vectors for this path.
  *
  * Defined behaviour where the reference is undefined (documented, shared with
