@@ -245,8 +245,12 @@ typedef struct pvvc_color_job {
                                    SYMBOL_RANGE (coefficient outside int32), CAPACITY (out_cap) */
 } pvvc_color_job;
 /* Encodes every job's colours (one launch); blocks until the bytes are back.
- * kernel_ms (optional) receives the encode kernel's time. */
+ * kernel_ms (optional) receives the encode kernel's time.  The device image
+ * and pinned staging buffers are kept for the next call (grow-only); calls
+ * from several threads are serialised. */
 int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms);
+/* Frees the encoder's kept buffers (the next pvvc_encode_colors makes them again). */
+void pvvc_encoder_release(void);
 
 /* ------------------------------------------------------------------------ */
 /* Quality metric: symmetric nearest-neighbour MSE of two coloured clouds,   */

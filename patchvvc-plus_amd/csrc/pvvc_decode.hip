@@ -18,6 +18,7 @@
 #include <mutex>
 #include <new>
 #include <numeric>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -2696,16 +2697,87 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
     return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
 }
 
+}  // extern "C"
+
 /* ---- forward path (colour encoder) ---- */
+namespace {
+
+/* Buffers of the colour encoder, kept between calls and only ever grown: the
+ * device image, a pinned host copy of its input part (same offsets) and a
+ * pinned landing area for the packed output bytes.  Calls are serialised by
+ * g_enc_mu. */
+struct EncBuffers {
+    uint8_t* dev = nullptr;
+    uint64_t dev_cap = 0;
+    uint8_t* pin = nullptr;
+    uint64_t pin_cap = 0;
+    uint8_t* pin_out = nullptr;
+    uint64_t pin_out_cap = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+EncBuffers g_enc;
+std::mutex g_enc_mu;
+
+uint64_t grow_size(uint64_t need) { return ((need + need / 4) + 0xFFFFF) & ~uint64_t(0xFFFFF); }
+
+bool enc_grow_dev(uint64_t need) {
+    if (need <= g_enc.dev_cap) return true;
+    if (g_enc.dev) {
+        hipStreamSynchronize(g_stream);
+        hipFree(g_enc.dev);
+        g_enc.dev = nullptr;
+        g_enc.dev_cap = 0;
+    }
+    const uint64_t cap = grow_size(need);
+    if (!hip_ok(hipMalloc(&g_enc.dev, cap))) return false;
+    g_enc.dev_cap = cap;
+    return true;
+}
+
+bool enc_grow_pinned(uint8_t*& p, uint64_t& cap_now, uint64_t need) {
+    if (need <= cap_now) return true;
+    if (p) {
+        hipStreamSynchronize(g_stream);
+        hipHostFree(p);
+        p = nullptr;
+        cap_now = 0;
+    }
+    const uint64_t cap = grow_size(need);
+    if (!hip_ok(hipHostMalloc(reinterpret_cast<void**>(&p), cap, hipHostMallocDefault))) return false;
+    cap_now = cap;
+    return true;
+}
+
+int host_threads(int tasks) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::max(1, std::min({tasks, 8, hw ? static_cast<int>(hw) : 1}));
+}
+
+}  // namespace
+
+extern "C" {
+
+/* Steps of one call:
+ *  1. layout: device image [EncJobDev x n][results][offsets][inputs: geo |
+ *     yuv per job][outputs][workspaces]; the pinned buffer holds the first
+ *     four parts at the same offsets;
+ *  2. host threads copy the jobs' inputs into the pinned buffer, in chunks of
+ *     consecutive jobs; each finished chunk goes to the device by an
+ *     asynchronous copy while the next ones are filled;
+ *  3. one encode launch (one wavefront per job), the byte counts back;
+ *  4. a gather launch packs the variable-length outputs into the (now free)
+ *     input region, one copy brings them into pinned memory, and host threads
+ *     scatter them into the jobs' buffers. */
 int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     if (!jobs || n_jobs <= 0) return PVVC_ERR_INVALID_ARG;
     rt_init_kernels();
+    std::lock_guard<std::mutex> lk(g_enc_mu);
     auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
-    /* device image: [EncJobDev x n][results][offsets][inputs: geo | yuv per job][outputs and workspaces] */
     const uint64_t o_jobs = 0, o_res = al(sizeof(EncJobDev) * n_jobs), o_at = al(o_res + 8ull * n_jobs);
-    uint64_t at = al(o_at + 8ull * n_jobs);
+    const uint64_t in_begin = al(o_at + 8ull * n_jobs);
+    uint64_t at = in_begin;
     std::vector<uint64_t> o_geo(n_jobs), o_yuv(n_jobs), o_out(n_jobs), o_ws(n_jobs), ws_cap(n_jobs), out_cap(n_jobs);
-    /* inputs (geometry, leaf YUV) of all jobs first, so that one copy covers them */
+    /* inputs (geometry, leaf YUV) of all jobs first, in job order */
     for (int i = 0; i < n_jobs; ++i) {
         const pvvc_color_job& j = jobs[i];
         if (!j.geometry || j.geometry_size < 25 || !j.yuv || j.n == 0 || !j.out || j.qp == 0)
@@ -2726,53 +2798,77 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
         at += ws_cap[i];
     }
     const uint64_t total = at;
-    uint8_t* dev = nullptr;
-    if (!hip_ok(hipMalloc(&dev, total))) return PVVC_ERR_HIP;
-    /* [jobs][results][offsets][inputs]: one host image, one copy; every byte
-     * the kernel reads is written below (alignment gaps are never read) */
-    std::unique_ptr<uint8_t[]> hbuf(new (std::nothrow) uint8_t[in_end]);
-    if (!hbuf) {
-        hipFree(dev);
-        return PVVC_ERR_CAPACITY;
+    if (!enc_grow_dev(total) || !enc_grow_pinned(g_enc.pin, g_enc.pin_cap, in_end)) return PVVC_ERR_HIP;
+    if (!g_enc.e0 && !(hip_ok(hipEventCreate(&g_enc.e0)) && hip_ok(hipEventCreate(&g_enc.e1)))) return PVVC_ERR_HIP;
+    uint8_t* const dev = g_enc.dev;
+    uint8_t* const h = g_enc.pin;
+    {
+        EncJobDev* jd = reinterpret_cast<EncJobDev*>(h + o_jobs);
+        for (int i = 0; i < n_jobs; ++i) {
+            EncJobDev d;
+            d.geo = dev + o_geo[i];
+            d.yuv = reinterpret_cast<const float*>(dev + o_yuv[i]);
+            d.out = dev + o_out[i];
+            d.ws = dev + o_ws[i];
+            d.ws_cap = ws_cap[i];
+            d.geo_size = jobs[i].geometry_size;
+            d.n = jobs[i].n;
+            d.out_cap = static_cast<uint32_t>(out_cap[i]);
+            d.qp = jobs[i].qp;
+            jd[i] = d;
+        }
     }
-    uint8_t* const h = hbuf.get();
-    std::vector<EncJobDev> jd(n_jobs);
-    for (int i = 0; i < n_jobs; ++i) {
-        EncJobDev& d = jd[i];
-        d.geo = dev + o_geo[i];
-        d.yuv = reinterpret_cast<const float*>(dev + o_yuv[i]);
-        d.out = dev + o_out[i];
-        d.ws = dev + o_ws[i];
-        d.ws_cap = ws_cap[i];
-        d.geo_size = jobs[i].geometry_size;
-        d.n = jobs[i].n;
-        d.out_cap = static_cast<uint32_t>(out_cap[i]);
-        d.qp = jobs[i].qp;
+    bool good = hip_ok(hipMemcpyAsync(dev + o_jobs, h + o_jobs, o_res, hipMemcpyHostToDevice, g_stream));
+    /* chunks of consecutive jobs of about 4 MiB of input each (at most 64) */
+    const uint64_t in_bytes = in_end - in_begin;
+    const int n_chunks = static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>({64, uint64_t(n_jobs),
+                                                                                     in_bytes >> 22})));
+    std::vector<int> first(n_chunks + 1, n_jobs);
+    for (int c = 0, i = 0; c < n_chunks; ++c) {
+        first[c] = i;
+        const uint64_t stop = in_begin + in_bytes * (c + 1) / n_chunks;
+        while (i < n_jobs && (o_geo[i] < stop || i == first[c])) ++i;
+        if (c == n_chunks - 1) i = n_jobs;
     }
-    std::memcpy(h + o_jobs, jd.data(), sizeof(EncJobDev) * n_jobs);
-    for (int i = 0; i < n_jobs; ++i) {
-        std::memcpy(h + o_geo[i], jobs[i].geometry, jobs[i].geometry_size);
-        std::memcpy(h + o_yuv[i], jobs[i].yuv, 12ull * jobs[i].n);
+    std::unique_ptr<std::atomic<int>[]> done(new std::atomic<int>[n_chunks]);
+    for (int c = 0; c < n_chunks; ++c) done[c].store(0);
+    auto fill = [&](int c) {
+        for (int i = first[c]; i < first[c + 1]; ++i) {
+            std::memcpy(h + o_geo[i], jobs[i].geometry, jobs[i].geometry_size);
+            std::memcpy(h + o_yuv[i], jobs[i].yuv, 12ull * jobs[i].n);
+        }
+        done[c].store(1, std::memory_order_release);
+    };
+    const int T = host_threads(n_chunks);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; ++t)
+        pool.emplace_back([&, t] {
+            for (int c = t; c < n_chunks; c += T) fill(c);
+        });
+    /* chunk c's bytes are [start(c), start(c + 1)); a chunk may be empty when
+     * the ones before it took every job */
+    auto start = [&](int c) { return c == 0 ? in_begin : first[c] < n_jobs ? o_geo[first[c]] : in_end; };
+    for (int c = 0; c < n_chunks; ++c) {
+        while (!done[c].load(std::memory_order_acquire)) std::this_thread::yield();
+        const uint64_t b = start(c), e = start(c + 1);
+        if (good && e > b) good = hip_ok(hipMemcpyAsync(dev + b, h + b, e - b, hipMemcpyHostToDevice, g_stream));
     }
-    bool good = hip_ok(hipMemcpy(dev, h, in_end, hipMemcpyHostToDevice));
+    for (auto& th : pool) th.join();
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(16 * 1024));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
     std::vector<int64_t> res(n_jobs, 0);
     if (good) {
-        hipEventRecord(e0, g_stream);
+        hipEventRecord(g_enc.e0, g_stream);
         hipLaunchKernelGGL(encode_kernel, dim3(n_jobs), dim3(64), lds, g_stream,
                            reinterpret_cast<const EncJobDev*>(dev + o_jobs), static_cast<uint32_t>(n_jobs), lds,
                            reinterpret_cast<int64_t*>(dev + o_res));
-        hipEventRecord(e1, g_stream);
+        hipEventRecord(g_enc.e1, g_stream);
         good = hip_ok(hipGetLastError()) &&
                hip_ok(hipMemcpyAsync(res.data(), dev + o_res, 8ull * n_jobs, hipMemcpyDeviceToHost, g_stream)) &&
                hip_ok(hipStreamSynchronize(g_stream));
     }
     if (good && kernel_ms) {
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        hipEventElapsedTime(&ms, g_enc.e0, g_enc.e1);
         *kernel_ms = ms;
     }
     /* compact the variable-length outputs, one copy back */
@@ -2782,35 +2878,58 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
         pos[i] = packed_bytes;
         if (res[i] > 0) packed_bytes += static_cast<uint64_t>(res[i]);
     }
-    uint8_t* packed = nullptr;
+    uint8_t* spill = nullptr; /* only when the outputs outgrow the input region */
     if (good && packed_bytes) {
-        good = hip_ok(hipMalloc(&packed, packed_bytes)) &&
+        uint8_t* packed = dev + in_begin;
+        if (packed_bytes > in_bytes) good = hip_ok(hipMalloc(&spill, packed_bytes)), packed = spill;
+        good = good && enc_grow_pinned(g_enc.pin_out, g_enc.pin_out_cap, packed_bytes) &&
                hip_ok(hipMemcpyAsync(dev + o_at, pos.data(), 8ull * n_jobs, hipMemcpyHostToDevice, g_stream));
         if (good) {
             hipLaunchKernelGGL(gather_kernel, dim3(n_jobs), dim3(256), 0, g_stream,
                                reinterpret_cast<const EncJobDev*>(dev + o_jobs),
                                reinterpret_cast<const int64_t*>(dev + o_res),
                                reinterpret_cast<const uint64_t*>(dev + o_at), static_cast<uint32_t>(n_jobs), packed);
-            std::vector<uint8_t> hp(packed_bytes);
             good = hip_ok(hipGetLastError()) &&
-                   hip_ok(hipMemcpyAsync(hp.data(), packed, packed_bytes, hipMemcpyDeviceToHost, g_stream)) &&
+                   hip_ok(hipMemcpyAsync(g_enc.pin_out, packed, packed_bytes, hipMemcpyDeviceToHost, g_stream)) &&
                    hip_ok(hipStreamSynchronize(g_stream));
-            for (int i = 0; good && i < n_jobs; ++i)
-                if (res[i] > 0) {
-                    if (static_cast<uint64_t>(res[i]) > jobs[i].out_cap) res[i] = -PVVC_ERR_CAPACITY;
-                    else std::memcpy(jobs[i].out, hp.data() + pos[i], static_cast<size_t>(res[i]));
-                }
+        }
+        if (good) {
+            const uint8_t* hp = g_enc.pin_out;
+            auto scatter = [&](int t, int T2) {
+                for (int i = static_cast<int>(int64_t(n_jobs) * t / T2); i < int64_t(n_jobs) * (t + 1) / T2; ++i)
+                    if (res[i] > 0) {
+                        if (static_cast<uint64_t>(res[i]) > jobs[i].out_cap) res[i] = -PVVC_ERR_CAPACITY;
+                        else std::memcpy(jobs[i].out, hp + pos[i], static_cast<size_t>(res[i]));
+                    }
+            };
+            const int T2 = host_threads(static_cast<int>(std::min<uint64_t>(packed_bytes >> 20, 64) + 1));
+            std::vector<std::thread> sp;
+            for (int t = 1; t < T2; ++t) sp.emplace_back(scatter, t, T2);
+            scatter(0, T2);
+            for (auto& th : sp) th.join();
         }
     }
     for (int i = 0; i < n_jobs; ++i) {
         jobs[i].out_size = res[i] > 0 ? static_cast<uint32_t>(res[i]) : 0u;
         jobs[i].status = !good ? PVVC_ERR_HIP : (res[i] > 0 ? PVVC_OK : static_cast<int>(-res[i]));
     }
-    if (packed) hipFree(packed);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(dev);
+    if (spill) {
+        hipStreamSynchronize(g_stream);
+        hipFree(spill);
+    }
     return good ? PVVC_OK : PVVC_ERR_HIP;
+}
+
+/* Frees the colour encoder's kept buffers (they are made again on the next call). */
+void pvvc_encoder_release(void) {
+    std::lock_guard<std::mutex> lk(g_enc_mu);
+    if (g_stream) hipStreamSynchronize(g_stream);
+    if (g_enc.dev) hipFree(g_enc.dev);
+    if (g_enc.pin) hipHostFree(g_enc.pin);
+    if (g_enc.pin_out) hipHostFree(g_enc.pin_out);
+    if (g_enc.e0) hipEventDestroy(g_enc.e0);
+    if (g_enc.e1) hipEventDestroy(g_enc.e1);
+    g_enc = EncBuffers{};
 }
 
 /* Autotuner: coordinate descent over the launch knobs on a real batch.

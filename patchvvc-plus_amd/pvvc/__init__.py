@@ -25,7 +25,7 @@ __all__ = [
     "PvvcError", "StagedFrame", "launch_dropin", "synchronize", "last_kernel_ms", "last_launch_times",
     "STATUS", "Frame", "DeviceBuffer", "State", "Decoder", "Batch", "decode_dropin", "synth_stream",
     "SYNTH_PRESETS", "Points", "hip_lib", "enc_lib", "MODE_PER_FRAME", "MODE_PERSISTENT", "MODE_GRAPH",
-    "OUT_NATIVE", "OUT_POINTS", "device_info",
+    "OUT_NATIVE", "OUT_POINTS", "device_info", "encode_colors_packed",
 ]
 
 STATUS = {
@@ -448,27 +448,51 @@ def encode_colors(jobs: Sequence[tuple[bytes, np.ndarray, int]]) -> tuple[list[b
     yuvs = [np.ascontiguousarray(j[1], np.float32).reshape(-1, 3) for j in jobs]
     gsz = np.fromiter((len(g) for g in geos), np.int64, n)
     npts = np.fromiter((len(y) for y in yuvs), np.int64, n)
-    goff = np.concatenate([[0], np.cumsum(gsz)[:-1]]).astype(np.int64)
-    poff = np.concatenate([[0], np.cumsum(npts)[:-1]]).astype(np.int64)
+    qp = np.fromiter((int(j[2]) for j in jobs), np.uint8, n)
+    out_all, ooff, sizes, status, ms = encode_colors_packed(
+        np.frombuffer(b"".join(geos), np.uint8), gsz, np.concatenate(yuvs), npts, qp)
+    outs = [out_all[o:o + z].tobytes() for o, z in zip(ooff.tolist(), sizes.tolist())]
+    return outs, status.tolist(), ms
+
+
+def encode_colors_packed(geo_all: np.ndarray, geo_size: np.ndarray, yuv_all: np.ndarray, npts: np.ndarray,
+                         qp: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray, float]:
+    """GPU colour encoder over packed inputs: job i's octree bytes are the next
+    geo_size[i] bytes of geo_all, its leaf YUV the next npts[i] rows of
+    yuv_all (float32 [N, 3]), its quantisation step qp[i].  Returns
+    (out_all, out_off, out_size, status, kernel ms): job i's RLGR bytes are
+    out_all[out_off[i]: out_off[i] + out_size[i]].  The job table is filled by
+    array arithmetic alone, and the output buffer is not cleared (the library
+    writes exactly out_size[i] bytes per job)."""
+    geo_all = np.ascontiguousarray(geo_all, np.uint8)
+    yuv_all = np.ascontiguousarray(yuv_all, np.float32).reshape(-1, 3)
+    gsz = np.asarray(geo_size, np.int64)
+    npts = np.asarray(npts, np.int64)
+    n = len(npts)
+    if n == 0 or len(gsz) != n or len(qp) != n:
+        raise PvvcError(8, "encode_colors_packed: job arrays of different lengths (or none)")
+    if int(gsz.sum()) > geo_all.size or int(npts.sum()) > len(yuv_all):
+        raise PvvcError(8, "encode_colors_packed: sizes exceed the packed inputs")
+    goff = np.zeros(n, np.int64)
+    np.cumsum(gsz[:-1], out=goff[1:])
+    poff = np.zeros(n, np.int64)
+    np.cumsum(npts[:-1], out=poff[1:])
     cap = 24 * npts + 64
-    ooff = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
-    geo_all = np.frombuffer(b"".join(geos), np.uint8)
-    yuv_all = np.concatenate(yuvs) if n else np.zeros((0, 3), np.float32)
-    out_all = np.zeros(int(cap.sum()) if n else 1, np.uint8)
+    ooff = np.zeros(n, np.int64)
+    np.cumsum(cap[:-1], out=ooff[1:])
+    out_all = np.empty(int(cap.sum()), np.uint8)
     table = np.zeros(n, _color_job_dtype())
     table["geometry"] = geo_all.ctypes.data + goff
     table["geometry_size"] = gsz
     table["yuv"] = yuv_all.ctypes.data + 12 * poff
     table["n"] = npts
-    table["qp"] = np.fromiter((int(j[2]) for j in jobs), np.uint8, n)
+    table["qp"] = qp
     table["out"] = out_all.ctypes.data + ooff
     table["out_cap"] = cap
     ms = C.c_double()
     _check(hip_lib().pvvc_encode_colors(C.cast(table.ctypes.data, C.c_void_p), n, C.byref(ms)),
            "pvvc_encode_colors")
-    sizes = table["out_size"]
-    outs = [out_all[o:o + z].tobytes() for o, z in zip(ooff.tolist(), sizes.tolist())]
-    return outs, table["status"].tolist(), float(ms.value)
+    return out_all, ooff, table["out_size"].astype(np.int64), table["status"].copy(), float(ms.value)
 
 
 def encode_colors_cpu_on_tree(geo: bytes, leaf_yuv: np.ndarray, qp: int) -> bytes:

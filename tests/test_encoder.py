@@ -118,3 +118,50 @@ def test_gpu_encoder_chunk_boundaries(pvvc):
     assert status == [0] * len(jobs), status
     for o, r in zip(outs, refs):
         assert o == r, f"{len(r)} bytes (mod 256 = {len(r) % 256})"
+
+
+def test_packed_encoder_rejects_inconsistent_jobs(pvvc):
+    """Argument checks of the packed entry (host side, before any GPU call)."""
+    geo = np.zeros(30, np.uint8)
+    yuv = np.zeros((4, 3), np.float32)
+    with pytest.raises(pvvc.PvvcError):
+        pvvc.encode_colors_packed(geo, [30], yuv, [4, 1], np.array([1], np.uint8))
+    with pytest.raises(pvvc.PvvcError):
+        pvvc.encode_colors_packed(geo, [31], yuv, [4], np.array([1], np.uint8))
+    with pytest.raises(pvvc.PvvcError):
+        pvvc.encode_colors_packed(geo, [30], yuv, [5], np.array([1], np.uint8))
+
+
+@pytest.mark.gpu
+def test_gpu_encoder_kept_buffers(pvvc):
+    """Calls in a row reuse the library's device and pinned buffers: a large
+    call (~1.2 M leaves, ~14 MB of input, several staging chunks), a smaller
+    one, the large one again in reverse order, then again after the buffers
+    are released.  Every stream equals the CPU writer's bytes."""
+    jobs, refs = _jobs(pvvc, seed=31, count=40)
+    reps = 20
+    big, big_refs = jobs * reps, refs * reps
+    assert sum(len(j[1]) for j in big) * 12 > (8 << 20)
+    small, small_refs = _boundary_jobs(pvvc)
+    for name, js, rs in [("big", big, big_refs), ("small", small, small_refs),
+                         ("big reversed", big[::-1], big_refs[::-1]), ("small", small, small_refs)]:
+        outs, status, _ = pvvc.encode_colors(js)
+        assert status == [0] * len(js), name
+        bad = [i for i, (o, r) in enumerate(zip(outs, rs)) if o != r]
+        assert not bad, f"{name}: {len(bad)} streams differ, first {bad[:5]}"
+    pvvc.hip_lib().pvvc_encoder_release()
+    outs, status, _ = pvvc.encode_colors(small)
+    assert status == [0] * len(small) and outs == small_refs
+
+
+@pytest.mark.gpu
+def test_gpu_packed_encoder_equals_job_list(pvvc):
+    jobs, refs = _jobs(pvvc, seed=41, count=24)
+    geo_all = np.frombuffer(b"".join(bytes(j[0]) for j in jobs), np.uint8)
+    gsz = [len(j[0]) for j in jobs]
+    yuv_all = np.concatenate([j[1] for j in jobs])
+    npts = [len(j[1]) for j in jobs]
+    qp = np.array([j[2] for j in jobs], np.uint8)
+    out_all, off, size, status, ms = pvvc.encode_colors_packed(geo_all, gsz, yuv_all, npts, qp)
+    assert (status == 0).all() and ms > 0
+    assert [out_all[o:o + z].tobytes() for o, z in zip(off, size)] == refs
