@@ -214,6 +214,19 @@ int pvvc_decode_batch(pvvc_decoder* dec, pvvc_batch* batch, int mode, int output
                       Points* points);
 /* Per-item status of the last decode of this batch (synchronises). */
 int pvvc_batch_status(pvvc_batch* batch, int* status, int n);
+
+/* One-call decodes (pack, decode, wait): the per-frame step of the client
+ * decoder thread (Manager::StartDecoder, src/client/manager.cc:257, which
+ * calls Render::CUDADecode, src/cuda/render/render.cc:270-289) and a whole
+ * GoP in one persistent launch.  Outputs are device buffers as for
+ * pvvc_decode_batch; frame f of a GoP starts after the points of frames
+ * [0, f).  Returns the call's own error, else the first non-OK slice status
+ * (frames in order, slices in file order), else PVVC_OK; `status`
+ * (optional, status_cap entries) receives every slice's status. */
+int pvvc_decode_frame(pvvc_decoder* dec, const pvvc_frame* frame, int output, float* xyz, uint8_t* rgb,
+                      Points* points, int* status, int status_cap);
+int pvvc_decode_gop(pvvc_decoder* dec, const pvvc_frame* const* frames, int n_frames, int output, float* xyz,
+                    uint8_t* rgb, Points* points, int* status, int status_cap);
 /* Resets every patch state slot (drops GoP state). */
 int pvvc_decoder_reset(pvvc_decoder* dec);
 

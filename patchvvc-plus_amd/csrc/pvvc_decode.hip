@@ -2697,6 +2697,44 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
     return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
 }
 
+/* Synchronous one-call forms of the batch API: pack, decode, wait, free.
+ * The return value is the call's own error, else the first non-OK slice
+ * status in item order (frames in order, each frame's slices in file order),
+ * else PVVC_OK; `status` (optional) receives every slice's status. */
+namespace {
+int decode_once(pvvc_decoder* d, const pvvc_frame* const* frames, int n_frames, int mode, int output, float* xyz,
+                uint8_t* rgb, Points* points, int* status, int status_cap) {
+    if (!d || !frames || n_frames <= 0 || (status && status_cap < 0)) return PVVC_ERR_INVALID_ARG;
+    pvvc_batch* b = nullptr;
+    int rc = pvvc_batch_create(d, frames, n_frames, &b);
+    if (rc != PVVC_OK) return rc;
+    rc = pvvc_decode_batch(d, b, mode, output, xyz, rgb, points);
+    std::vector<int> st(b->n_items, PVVC_OK);
+    if (rc == PVVC_OK) rc = pvvc_batch_status(b, st.data(), static_cast<int>(st.size()));
+    else hipStreamSynchronize(g_stream);
+    if (rc == PVVC_OK) {
+        if (status) std::copy(st.begin(), st.begin() + std::min<size_t>(st.size(), status_cap), status);
+        for (int s : st)
+            if (s != PVVC_OK) {
+                rc = s;
+                break;
+            }
+    }
+    pvvc_batch_free(b);
+    return rc;
+}
+}  // namespace
+
+int pvvc_decode_frame(pvvc_decoder* dec, const pvvc_frame* frame, int output, float* xyz, uint8_t* rgb,
+                      Points* points, int* status, int status_cap) {
+    return decode_once(dec, &frame, 1, PVVC_MODE_PER_FRAME, output, xyz, rgb, points, status, status_cap);
+}
+
+int pvvc_decode_gop(pvvc_decoder* dec, const pvvc_frame* const* frames, int n_frames, int output, float* xyz,
+                    uint8_t* rgb, Points* points, int* status, int status_cap) {
+    return decode_once(dec, frames, n_frames, PVVC_MODE_PERSISTENT, output, xyz, rgb, points, status, status_cap);
+}
+
 }  // extern "C"
 
 /* ---- forward path (colour encoder) ---- */

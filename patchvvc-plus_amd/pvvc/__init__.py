@@ -413,6 +413,33 @@ class Decoder:
                                                p(out.pts), reps, C.byref(t), C.byref(ms)), "pvvc_decoder_autotune")
         return {k: getattr(t, k) for k, _ in Tuning._fields_}, float(ms.value)
 
+    def outputs_for(self, frames: Sequence[Frame], output: int = OUT_NATIVE) -> Outputs:
+        n = sum(f.points for f in frames)
+        m = max(1, n)
+        if output == OUT_NATIVE:
+            return Outputs(xyz=DeviceBuffer(12 * m), rgb=DeviceBuffer(3 * m), points=n)
+        return Outputs(pts=DeviceBuffer(24 * m), points=n)
+
+    def decode_frame(self, frame: Frame, out: Outputs) -> tuple[int, np.ndarray]:
+        """pvvc_decode_frame: one frame, synchronous.  Returns (code, per-slice
+        statuses); code is the first non-OK status or 0."""
+        st = np.zeros(max(1, frame.slice_cnt), np.int32)
+        p = lambda b: C.c_void_p(b.ptr) if b is not None else None  # noqa: E731
+        rc = hip_lib().pvvc_decode_frame(self.handle, frame.handle, OUT_NATIVE if out.xyz is not None else OUT_POINTS,
+                                         p(out.xyz), p(out.rgb), p(out.pts), st.ctypes.data, frame.slice_cnt)
+        return int(rc), st[:frame.slice_cnt]
+
+    def decode_gop(self, frames: Sequence[Frame], out: Outputs) -> tuple[int, np.ndarray]:
+        """pvvc_decode_gop: frames in decode order, one persistent launch,
+        synchronous.  Returns (code, per-slice statuses in frame order)."""
+        n = sum(f.slice_cnt for f in frames)
+        st = np.zeros(max(1, n), np.int32)
+        arr = (C.c_void_p * len(frames))(*[f.handle.value for f in frames])
+        p = lambda b: C.c_void_p(b.ptr) if b is not None else None  # noqa: E731
+        rc = hip_lib().pvvc_decode_gop(self.handle, arr, len(frames), OUT_NATIVE if out.xyz is not None else OUT_POINTS,
+                                       p(out.xyz), p(out.rgb), p(out.pts), st.ctypes.data, n)
+        return int(rc), st[:n]
+
     def decode_host(self, batch: Batch, mode: int = MODE_PERSISTENT, output: int = OUT_NATIVE):
         """Decodes and reads back: (xyz f32[N,3], rgb u8[N,3]) or Points[N]."""
         out = self.outputs(batch, output)

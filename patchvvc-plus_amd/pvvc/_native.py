@@ -134,6 +134,8 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_batch_state_bytes", C.c_int64, c_vp)
     _sig(L, "pvvc_batch_status", C.c_int, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_decode_batch", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp)
+    _sig(L, "pvvc_decode_frame", C.c_int, c_vp, c_vp, C.c_int, c_vp, c_vp, c_vp, c_vp, C.c_int)
+    _sig(L, "pvvc_decode_gop", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_encode_colors", C.c_int, c_vp, C.c_int, C.POINTER(C.c_double))
     _sig(L, "pvvc_encoder_release", None)
     _sig(L, "pvvc_cloud_mse", C.c_int, c_vp, c_vp, C.c_uint64, c_vp, c_vp, C.c_uint64, c_vp, c_vp, C.POINTER(Mse),

@@ -161,3 +161,66 @@ def test_dropin_statuses_and_healthy_slices(pvvc, oracle, faulty):
             a, c = off[s], off[s + 1]
             np.testing.assert_array_equal(xyz[a:c], refs[k]["xyz"][a:c])
             np.testing.assert_array_equal(rgbf[a:c], refs[k]["rgbf"][a:c] / np.float32(255))
+
+
+@pytest.mark.gpu
+def test_one_call_decodes_report_statuses(pvvc, oracle, faulty):
+    """pvvc_decode_frame / pvvc_decode_gop: per-slice statuses equal the
+    oracle's, healthy slices decode bit-exactly, and the return code is the
+    first failing slice's status."""
+    paths, expect = faulty
+    refs = _oracle(oracle, paths)
+    frames = [pvvc.Frame.load(str(p)) for p in paths]
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames) + 1
+    _say("one-call decode_frame")
+    dec = pvvc.Decoder(n_patches, max_pts)
+    for k, f in enumerate(frames):
+        out = dec.outputs_for([f])
+        rc, st = dec.decode_frame(f, out)
+        n = f.points
+        xyz = out.xyz.read(np.float32, 3 * n).reshape(n, 3)
+        rgb = out.rgb.read(np.uint8, 3 * n).reshape(n, 3)
+        _check_frame(k, st, refs[k], expect[k], xyz, rgb, 0)
+        assert rc == st[np.flatnonzero(st)[0]]
+    _say("one-call decode_gop")
+    dec = pvvc.Decoder(n_patches, max_pts)
+    out = dec.outputs_for(frames)
+    rc, st = dec.decode_gop(frames, out)
+    n = sum(f.points for f in frames)
+    xyz = out.xyz.read(np.float32, 3 * n).reshape(n, 3)
+    rgb = out.rgb.read(np.uint8, 3 * n).reshape(n, 3)
+    first, off = 0, 0
+    for k, f in enumerate(frames):
+        _check_frame(k, st[first:first + f.slice_cnt], refs[k], expect[k], xyz, rgb, off)
+        first += f.slice_cnt
+        off += f.points
+    assert rc == st[np.flatnonzero(st)[0]]
+
+
+@pytest.mark.gpu
+def test_one_call_gop_equals_batch_api(pvvc, c0_stream):
+    frames = [pvvc.Frame.parse(b) for b in c0_stream]
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames) + 1
+    _say("one-call gop vs batch, c0")
+    dec = pvvc.Decoder(n_patches, max_pts)
+    xyz_b, rgb_b = dec.decode_host(dec.batch(frames), pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+    dec2 = pvvc.Decoder(n_patches, max_pts)
+    out = dec2.outputs_for(frames, pvvc.OUT_POINTS)
+    rc, st = dec2.decode_gop(frames, out)
+    assert rc == 0 and (st == 0).all()
+    pts = out.pts.read(pvvc.POINTS_DTYPE, len(xyz_b))
+    np.testing.assert_array_equal(np.stack([pts["x"], pts["y"], pts["z"]], 1), xyz_b)
+    # Points carries unrounded, unclamped float/255 colours; the native output
+    # rounds and saturates them
+    rgb_p = np.clip(np.stack([pts["r"], pts["g"], pts["b"]], 1) * 255, 0, 255)
+    assert np.abs(rgb_p - rgb_b.astype(np.float32)).max() <= 0.5 + 1e-3
+    dec3 = pvvc.Decoder(n_patches, max_pts)
+    for k, f in enumerate(frames):
+        out = dec3.outputs_for([f])
+        rc, st = dec3.decode_frame(f, out)
+        assert rc == 0 and (st == 0).all()
+        o = int(sum(g.points for g in frames[:k]))
+        np.testing.assert_array_equal(out.xyz.read(np.float32, 3 * f.points).reshape(-1, 3), xyz_b[o:o + f.points])
+        np.testing.assert_array_equal(out.rgb.read(np.uint8, 3 * f.points).reshape(-1, 3), rgb_b[o:o + f.points])
