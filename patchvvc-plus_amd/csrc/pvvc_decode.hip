@@ -157,7 +157,12 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * the line (15/17) to raise k_P by 3, lower k_RP and test the mode, and its
  * accumulator lane already holds 0.  k_RP's step (p ? p - 1 : -2, clamped to
  * [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the carry-in of
- * s_addc from the SCC of p << kR.  A run entry whose prefix has no one-bits
+ * s_addc from the SCC of p << kR.  The clamp to 128 never binds here: a code
+ * is decoded only when p + 1 + k_R <= 32 (run paths: p + k_R <= 29), and
+ * k_RP <= 4 k_R + 3, so the new k_RP is at most 126 - 3p.  Nor does the
+ * clamp to 0 on a non-zero k = 0 symbol: p = 0 with u != 0 needs k_R >= 1,
+ * i.e. k_RP >= 4.  Only the zero symbols and the run paths (where u = 0 is a
+ * valid symbol, 1) keep it.  A run entry whose prefix has no one-bits
  * (the common case) has its own path (30/40): its zero count is the k-bit
  * suffix alone and `neg` advances in place (undone on the out-of-line exits
  * 37/47); prefixes with one-bits take the general path (36/46, with the
@@ -195,9 +200,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
         "s_cbranch_scc0 15f\n\t"
-        "s_sub_i32 s90, s90, 2\n\t"
-        "s_max_i32 s90, s90, 0\n\t"
-        "s_min_i32 %[kRP], s90, 128\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
         "v_mov_b32 %[vt], s89\n\t"
@@ -226,9 +229,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
         "s_cbranch_scc0 17f\n\t"
-        "s_sub_i32 s90, s90, 2\n\t"
-        "s_max_i32 s90, s90, 0\n\t"
-        "s_min_i32 %[kRP], s90, 128\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
         "v_mov_b32 %[vt], s89\n\t"
@@ -347,8 +348,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s88, s88, 1\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
-        "s_max_i32 s91, s91, 0\n\t"
-        "s_min_i32 %[kRP], s91, 128\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
         "v_mov_b32 %[vt], s88\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
         "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
@@ -391,8 +391,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s88, s88, 1\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
-        "s_max_i32 s91, s91, 0\n\t"
-        "s_min_i32 %[kRP], s91, 128\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
         "v_mov_b32 %[vt], s88\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
         "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
@@ -468,8 +467,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s88, s88, 1\n\t"
         "s_sub_u32 %[kP], s87, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
-        "s_max_i32 s91, s91, 0\n\t"
-        "s_min_i32 %[kRP], s91, 128\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
         "s_mov_b32 %[neg], s83\n\t"
         "v_mov_b32 %[vt], s88\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
@@ -527,8 +525,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s88, s88, 1\n\t"
         "s_sub_u32 %[kP], s87, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
-        "s_max_i32 s91, s91, 0\n\t"
-        "s_min_i32 %[kRP], s91, 128\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
         "s_mov_b32 %[neg], s83\n\t"
         "v_mov_b32 %[vt], s88\n\t"
         "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
