@@ -292,18 +292,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 19b\n\t"
-        "17:\n\t"
-        "s_sub_i32 s90, s90, 2\n\t"
-        "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_add_u32 %[kP], %[kP], 3\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
-        "s_cbranch_scc1 20f\n\t"
-        "21:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc1 30f\n\t"
-        "s_branch 10b\n\t"
         "20:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
@@ -316,7 +304,18 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
-        "s_branch 21b\n\t"
+        "s_branch 21f\n\t"
+        "17:\n\t"
+        "s_sub_i32 s90, s90, 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "s_add_u32 %[kP], %[kP], 3\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 20b\n\t"
+        "21:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc0 10b\n\t"
         "30:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
         "s_not_b64 s[80:81], %[win]\n\t"
@@ -359,7 +358,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
-        "s_branch 40f\n\t"
         "40:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
         "s_not_b64 s[80:81], %[win]\n\t"
