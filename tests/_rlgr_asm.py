@@ -1,0 +1,334 @@
+"""CPU emulation of the decoder's hand-written scalar RLGR loop.
+
+The loop (`rlgr_fast_run` in patchvvc-plus_amd/csrc/pvvc_decode.hip) is GCN
+scalar assembly inside an inline-asm string.  This module reads that string
+from the source, executes it instruction by instruction with the ISA's
+semantics for the subset it uses (SCC rules included), and states the RLGR
+decoder independently (entropy_codec.cc:207-283: adaptive Golomb-Rice with
+run mode, L = 4, k_P from 0, k_RP from 8), so that a test can check every
+state the loop leaves against the decoder's definition.
+
+Register model: the operands %[win] (64-bit), %[nbx], %[wpos], %[kRP], %[kP],
+%[neg] (32-bit scalars); %[acc], %[vt] (per-lane vectors, read/write),
+%[cur], %[vrev] (per-lane inputs); s80-s93 scratch; vcc a 64-bit lane mask.
+"""
+from __future__ import annotations
+
+import re
+from pathlib import Path
+
+M32 = (1 << 32) - 1
+M64 = (1 << 64) - 1
+HIP_SOURCE = Path(__file__).resolve().parents[1] / "patchvvc-plus_amd" / "csrc" / "pvvc_decode.hip"
+
+
+def extract_loop(path: Path = HIP_SOURCE) -> list[str]:
+    """Instruction strings of the asm block in rlgr_fast_run, labels included."""
+    text = Path(path).read_text()
+    f = text.index("void rlgr_fast_run(")
+    a = text.index("asm volatile(", f)
+    b = text.index('\n        : [win]', a)
+    return re.findall(r'"([^"\n]*)\\n\\t"', text[a:b])
+
+
+class Loop:
+    def __init__(self, lines: list[str]):
+        self.ins = []
+        self.labels: dict[str, list[int]] = {}
+        for s in lines:
+            m = re.fullmatch(r"(\d+):", s.strip())
+            if m:
+                self.labels.setdefault(m.group(1), []).append(len(self.ins))
+                continue
+            op, _, rest = s.strip().partition(" ")
+            args = [x.strip() for x in rest.split(",")] if rest else []
+            self.ins.append((op, args))
+
+    def target(self, pc: int, ref: str) -> int:
+        n, d = ref[:-1], ref[-1]
+        pos = self.labels[n]
+        if d == "f":
+            return min(p for p in pos if p > pc)
+        return max(p for p in pos if p <= pc)
+
+    # ------------------------------------------------------------------
+    def run(self, st: dict, max_steps: int = 200000, visited: set | None = None) -> tuple[dict, int]:
+        s = {k: (list(v) if isinstance(v, list) else v) for k, v in st.items()}
+        sg = {}  # scratch SGPRs
+        scc = 0
+        vcc = 0
+
+        def rd(x):
+            if x.startswith("%["):
+                return s[x[2:-1]]
+            if x == "vcc":
+                return vcc
+            m = re.fullmatch(r"s\[(\d+):(\d+)\]", x)
+            if m:
+                lo, hi = int(m.group(1)), int(m.group(2))
+                assert hi == lo + 1
+                return (sg.get(lo, 0) & M32) | ((sg.get(hi, 0) & M32) << 32)
+            m = re.fullmatch(r"s(\d+)", x)
+            if m:
+                return sg.get(int(m.group(1)), 0)
+            return int(x, 0) & M32 if not x.startswith("-") else (int(x) & M32)
+
+        def wr(x, v, bits=32):
+            if x.startswith("%["):
+                name = x[2:-1]
+                s[name] = v & (M64 if name == "win" else M32)
+                return
+            m = re.fullmatch(r"s\[(\d+):(\d+)\]", x)
+            if m:
+                lo = int(m.group(1))
+                sg[lo] = v & M32
+                sg[lo + 1] = (v >> 32) & M32
+                return
+            m = re.fullmatch(r"s(\d+)", x)
+            assert m, x
+            sg[int(m.group(1))] = v & M32
+
+        def sx(v):
+            v &= M32
+            return v - (1 << 32) if v >> 31 else v
+
+        pc, steps = 0, 0
+        while pc < len(self.ins):
+            steps += 1
+            if steps > max_steps:
+                raise RuntimeError("no exit within the step bound")
+            op, a = self.ins[pc]
+            if visited is not None:
+                visited.add(pc)
+            nxt = pc + 1
+            if op == "s_branch":
+                nxt = self.target(pc, a[0])
+            elif op in ("s_cbranch_scc0", "s_cbranch_scc1"):
+                if scc == (1 if op.endswith("1") else 0):
+                    nxt = self.target(pc, a[0])
+            elif op == "s_nop":
+                pass
+            elif op == "s_mov_b32":
+                wr(a[0], rd(a[1]))
+            elif op == "s_not_b64":
+                v = ~rd(a[1]) & M64
+                wr(a[0], v)
+                scc = int(v != 0)
+            elif op == "s_flbit_i32_b64":
+                v = rd(a[1]) & M64
+                wr(a[0], (64 - v.bit_length()) if v else M32)
+            elif op == "s_add_u32":
+                v = rd(a[1]) + rd(a[2])
+                wr(a[0], v)
+                scc = int(v > M32)
+            elif op == "s_addc_u32":
+                v = rd(a[1]) + rd(a[2]) + scc
+                wr(a[0], v)
+                scc = int(v > M32)
+            elif op == "s_sub_u32":
+                x, y = rd(a[1]), rd(a[2])
+                wr(a[0], x - y)
+                scc = int(x < y)
+            elif op in ("s_add_i32", "s_sub_i32"):
+                x, y = sx(rd(a[1])), sx(rd(a[2]))
+                v = x + y if op == "s_add_i32" else x - y
+                wr(a[0], v)
+                scc = int(not (-(1 << 31) <= v < (1 << 31)))
+            elif op == "s_max_i32":
+                x, y = sx(rd(a[1])), sx(rd(a[2]))
+                wr(a[0], x if x > y else y)
+                scc = int(x > y)
+            elif op == "s_min_i32":
+                x, y = sx(rd(a[1])), sx(rd(a[2]))
+                wr(a[0], x if x < y else y)
+                scc = int(x < y)
+            elif op in ("s_lshl_b32", "s_lshr_b32"):
+                x, y = rd(a[1]) & M32, rd(a[2]) & 31
+                v = ((x << y) if op == "s_lshl_b32" else (x >> y)) & M32
+                wr(a[0], v)
+                scc = int(v != 0)
+            elif op in ("s_lshl_b64", "s_lshr_b64"):
+                x, y = rd(a[1]) & M64, rd(a[2]) & 63
+                v = ((x << y) if op == "s_lshl_b64" else (x >> y)) & M64
+                wr(a[0], v)
+                scc = int(v != 0)
+            elif op in ("s_or_b32", "s_and_b32"):
+                x, y = rd(a[1]) & M32, rd(a[2]) & M32
+                v = (x | y) if op == "s_or_b32" else (x & y)
+                wr(a[0], v)
+                scc = int(v != 0)
+            elif op == "s_or_b64":
+                v = (rd(a[1]) | rd(a[2])) & M64
+                wr(a[0], v)
+                scc = int(v != 0)
+            elif op.startswith("s_cmp_"):
+                x, y = rd(a[0]) & M32, rd(a[1]) & M32
+                scc = int({"s_cmp_gt_u32": x > y, "s_cmp_eq_u32": x == y, "s_cmp_lg_u32": x != y,
+                           "s_cmp_lt_u32": x < y, "s_cmp_ge_u32": x >= y}[op])
+            elif op == "v_mov_b32":
+                v = rd(a[1]) & M32
+                s[a[0][2:-1]] = [v] * 64
+            elif op == "v_cmp_eq_u32":
+                assert a[0] == "vcc"
+                x = rd(a[1]) & M32
+                vec = s[a[2][2:-1]]
+                vcc = sum(1 << l for l in range(64) if (vec[l] & M32) == x)
+            elif op == "v_cndmask_b32":
+                d, a0, a1 = s[a[1][2:-1]], s[a[2][2:-1]], a[3]
+                assert a1 == "vcc"
+                s[a[0][2:-1]] = [a0[l] if (vcc >> l) & 1 else d[l] for l in range(64)]
+            elif op == "v_readlane_b32":
+                wr(a[0], s[a[1][2:-1]][rd(a[2]) & 63])
+            else:
+                raise NotImplementedError(op)
+            pc = nxt
+        return s, steps
+
+
+# ----------------------------------------------------------------------
+# RLGR decoder (entropy_codec.cc:207-283), event by event over a bit list
+# ----------------------------------------------------------------------
+class Bits:
+    def __init__(self, bits: list[int], pos: int = 0):
+        self.b, self.pos = bits, pos
+
+    def bit(self) -> int:
+        v = self.b[self.pos]
+        self.pos += 1
+        return v
+
+    def read(self, n: int) -> int:
+        v = 0
+        for _ in range(n):
+            v = (v << 1) | self.bit()
+        return v
+
+    def gr(self, k: int) -> int:
+        p = 0
+        while p < 32 and self.b[self.pos] == 1:
+            self.pos += 1
+            p += 1
+        if p == 32:
+            return self.read(32)
+        self.pos += 1  # the terminating 0
+        return (p << k) + self.read(k)
+
+
+def adapt_krp(krp: int, p: int) -> int:
+    return min(krp + min(p, 200) - 1, 128) if p else max(krp - 2, 0)
+
+
+def events(bits: Bits, kP: int, kRP: int):
+    """Yields, per event, (symbols as (offset, raw value) list, symbols advanced,
+    kP, kRP) after it.  Raw values are what the loop stores: u for k = 0
+    symbols, u + 1 after a zero run; zeros of a run are not stored."""
+    while True:
+        k = kP >> 2
+        kR = kRP >> 2
+        if k == 0:
+            u = bits.gr(kR)
+            kRP = adapt_krp(kRP, u >> kR)
+            kP = max(kP - 1, 0) if u else kP + 3
+            yield [(0, u)], 1, kP, kRP
+        else:
+            m = 0
+            while bits.bit() == 1:
+                m += 1 << k
+                kP += 2
+                k = kP >> 2
+            m += bits.read(k)
+            u = bits.gr(kR)
+            kRP = adapt_krp(kRP, u >> kR)
+            kP = max(kP - 1, 0)
+            yield [(m, u + 1)], m + 1, kP, kRP
+
+
+def to_bits(data: bytes, pad_dwords: int = 80) -> list[int]:
+    out = []
+    for byte in bytes(data) + bytes(4 * pad_dwords):
+        out.extend((byte >> (7 - i)) & 1 for i in range(8))
+    return out
+
+
+def decode_boundaries(bits: list[int], nsym: int):
+    """(bit position, kP, kRP, symbol index) at every event boundary."""
+    b = Bits(bits)
+    kP, kRP, n = 0, 8, 0
+    out = [(0, 0, 8, 0)]
+    for _, adv, kP, kRP in events(b, kP, kRP):
+        n += adv
+        out.append((b.pos, kP, kRP, n))
+        if n >= nsym:
+            break
+    return out
+
+
+def make_state(bits: list[int], pos: int, kP: int, kRP: int, rng, nb: int, wpos: int, lim: int, nrel: int):
+    """Loop entry state at bit `pos`: nb window bits, then the stream's dwords
+    in lanes wpos & 63 .. 63 of `cur` (lane 0 would be a chunk rotation)."""
+    win = 0
+    for i in range(nb):
+        win = (win << 1) | bits[pos + i]
+    win <<= 64 - nb
+    l0 = wpos & 63
+    cur = [int(x) for x in rng.integers(0, 1 << 32, 64, dtype="uint64")]
+    q = pos + nb
+    if l0:
+        for lane in range(l0, 64):
+            v = 0
+            for i in range(32):
+                v = (v << 1) | bits[q + i]
+            cur[lane] = v
+            q += 32
+    acc = [int(x) for x in rng.integers(0, 1 << 20, 64)]
+    for lane in range(nrel, 64):
+        acc[lane] = 0
+    return {
+        "win": win, "nbx": (nb - 33) & M32, "wpos": wpos & M32, "kRP": kRP, "kP": kP,
+        "neg": (nrel - lim) & M32, "acc": acc, "vt": [0] * 64, "cur": cur,
+        "vrev": [(l - lim) & M32 for l in range(64)],
+    }
+
+
+def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> str | None:
+    """None when the loop's exit state st1 equals the decoder's state after a
+    whole number of events from st0; otherwise a description."""
+    def s32(v):
+        return v - (1 << 32) if v >> 31 else v
+    nb0, nb1 = s32(st0["nbx"]) + 33, s32(st1["nbx"]) + 33
+    dw = (st1["wpos"] - st0["wpos"]) & M32
+    consumed = nb0 + 32 * dw - nb1
+    lim = (-st0["vrev"][0]) & M32
+    nrel0, nrel1 = (st0["neg"] + lim) & M32, (st1["neg"] + lim) & M32
+    if not (0 <= nrel1 <= lim) or nrel1 < nrel0 or consumed < 0:
+        return f"bad exit counters nrel {nrel0}->{nrel1} lim {lim} consumed {consumed}"
+    # window contents: nb1 bits from pos + consumed, left aligned, zero below
+    want = 0
+    for i in range(nb1):
+        want = (want << 1) | bits[pos + consumed + i]
+    want = (want << (64 - nb1)) & M64 if nb1 > 0 else 0
+    if st1["win"] != want:
+        return f"window {st1['win']:016x} != {want:016x}"
+    b = Bits(bits, pos)
+    kP, kRP, n = st0["kP"], st0["kRP"], nrel0
+    acc = list(st0["acc"])
+    if consumed == 0:
+        ok = (nrel1 == nrel0 and st1["kP"] == kP and st1["kRP"] == kRP and st1["acc"] == acc)
+        return None if ok else "state changed without consuming bits"
+    for syms, adv, kP, kRP in events(b, kP, kRP):
+        for off, val in syms:
+            if n + off < 64:
+                acc[n + off] = val & M32
+        n += adv
+        used = b.pos - pos
+        if used == consumed:
+            if n != nrel1:
+                return f"symbol index {nrel1} != decoder {n}"
+            if st1["kP"] != kP or st1["kRP"] != kRP:
+                return f"kP/kRP {st1['kP']}/{st1['kRP']} != decoder {kP}/{kRP}"
+            if [x & M32 for x in st1["acc"]] != acc:
+                bad = [l for l in range(64) if (st1["acc"][l] & M32) != acc[l]]
+                return f"acc lanes differ: {bad[:8]}"
+            return None
+        if used > consumed:
+            return f"loop consumed {consumed} bits, not at an event boundary (decoder at {used})"

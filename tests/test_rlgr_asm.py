@@ -1,0 +1,91 @@
+"""The decoder's hand-written scalar RLGR loop, checked on the CPU.
+
+`rlgr_fast_run` (patchvvc-plus_amd/csrc/pvvc_decode.hip) is GCN scalar
+assembly.  _rlgr_asm.py executes its text instruction by instruction and
+states the RLGR decoder (entropy_codec.cc:207-283) independently.  Entry
+states are cut from real streams (the oracle's encoder over the crafted
+symbol kinds of the GPU stress test) at event boundaries, with random window
+fill, refill lane, chunk limit and symbol index, so that every exit (chunk
+complete, chunk rotation, long codes, run prefixes, runs crossing the chunk)
+is reached.  Each exit state must equal the decoder's state after a whole
+number of events: bits consumed, window contents, k_P, k_RP, the symbol index
+and every accumulator lane."""
+import numpy as np
+import pytest
+
+import _rlgr_asm as R
+from test_gpu_rlgr_stress import KINDS, _symbols
+
+
+def _cases(oracle, seed, streams_per_kind, states_per_stream):
+    rng = np.random.default_rng(seed)
+    for kind in KINDS:
+        for _ in range(streams_per_kind):
+            L = int(rng.integers(300, 5000))
+            sym = _symbols(kind, L, rng)
+            bits = R.to_bits(oracle.rlgr_encode(sym))
+            bnd = R.decode_boundaries(bits, L)
+            for _ in range(states_per_stream):
+                pos, kP, kRP, _n = bnd[int(rng.integers(0, len(bnd)))]
+                nb = int(rng.integers(33, 65))
+                wpos = int(rng.integers(0, 1 << 20))
+                lim = int(rng.integers(1, 65))
+                nrel = int(rng.integers(0, lim))
+                yield kind, bits, pos, R.make_state(bits, pos, kP, kRP, rng, nb, wpos, lim, nrel)
+
+
+def _check(oracle, lines, seed=5, streams_per_kind=2, states_per_stream=30):
+    loop = R.Loop(lines)
+    visited = set()
+    fails = []
+    for kind, bits, pos, st0 in _cases(oracle, seed, streams_per_kind, states_per_stream):
+        st1, _ = loop.run(st0, visited=visited)
+        err = R.check_against_decoder(bits, pos, st0, st1)
+        if err:
+            fails.append((kind, err))
+    return fails, len(visited), len(loop.ins)
+
+
+def test_decoder_statement_matches_oracle(oracle):
+    """The Python statement of the decoder reproduces the symbols (and so the
+    oracle's decode) of every crafted stream."""
+    rng = np.random.default_rng(1)
+    for kind in KINDS:
+        L = int(rng.integers(300, 4000))
+        sym = _symbols(kind, L, rng)
+        data = oracle.rlgr_encode(sym)
+        got, n = [], 0
+        for syms, adv, _, _ in R.events(R.Bits(R.to_bits(data)), 0, 8):
+            (m, raw), = syms
+            if adv > 1 or m:  # run: m zeros, then the symbol stored as u + 1
+                got += [0] * m
+            got.append((raw >> 1) ^ -(raw & 1))
+            n += adv
+            if n >= L:
+                break
+        np.testing.assert_array_equal(np.array(got[:L]), sym, err_msg=kind)
+        ref, st = oracle.rlgr_decode(data, L)
+        assert st == 0
+        np.testing.assert_array_equal(np.asarray(ref), sym, err_msg=kind)
+
+
+def test_scalar_loop_matches_decoder(oracle):
+    fails, covered, n_ins = _check(oracle, R.extract_loop())
+    assert not fails, fails[:5]
+    assert covered == n_ins, f"{n_ins - covered} of {n_ins} loop instructions never executed"
+
+
+@pytest.mark.parametrize("old,new", [
+    ("s_add_u32 %[kP], %[kP], 3", "s_add_u32 %[kP], %[kP], 2"),      # zero symbol: k_P step
+    ("s_sub_u32 s88, 63, s85", "s_sub_u32 s88, 62, s85"),            # remainder width
+    ("s_add_u32 s88, s88, 1", "s_add_u32 s88, s88, 0"),              # run symbol stored as u, not u + 1
+    ("s_max_i32 %[kRP], s91, 0", "s_mov_b32 %[kRP], s91"),           # run path: k_RP floor dropped
+    ("s_lshr_b32 s85, %[kP], 2", "s_lshr_b32 s85, %[kP], 1"),        # run exponent k = k_P / 4
+])
+def test_planted_loop_mistakes_are_caught(oracle, old, new):
+    """A mistake planted in the loop text (on the CPU) fails the check."""
+    lines = R.extract_loop()
+    i = next(j for j, s in enumerate(lines) if s.strip() == old)
+    lines = lines[:i] + [new] + lines[i + 1:]
+    fails, _, _ = _check(oracle, lines, streams_per_kind=1, states_per_stream=20)
+    assert fails, f"planted '{new}' went unnoticed"
