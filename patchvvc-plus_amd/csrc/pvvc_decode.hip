@@ -148,14 +148,16 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  *    (Unsign2Sign is applied when the chunk is stored);
  *  - the window takes 32 more bits when fewer than 33 remain; the borrow of
  *    nbx -= length says so.
- * The k = 0 and run cases are two loops (labels 10 and 30), each unrolled
- * twice and handing over to the other when k_P crosses 3, so that the common
- * path falls through: a taken scalar branch costs ~30 cycles on one
- * wavefront, an instruction ~4, and the kernel is bound by scalar issue, not
- * by one wavefront's latency.  A non-zero k = 0 symbol lowers k_P, so it
- * stays in the k = 0 loop without a mode test; a zero symbol (u = 0) leaves
- * the line (15/17) to raise k_P by 3, lower k_RP and test the mode, and its
- * accumulator lane already holds 0.  k_RP's step (p ? p - 1 : -2, clamped to
+ * The k = 0 and run cases are two loops, each unrolled four times (k = 0:
+ * 10, 16, 50, 54; run: 30, 40, 60, 70) and handing over to the other when
+ * k_P crosses 3, so that the common path falls through: a taken scalar
+ * branch costs a wavefront ~30 cycles against ~4 for an instruction, and a
+ * single slice's decode (the drop-in launch) is bound by that latency.  A
+ * non-zero k = 0 symbol lowers k_P, so it stays in the k = 0 loop without a
+ * mode test; a zero symbol (u = 0) leaves the line for the one zero block
+ * (17) to raise k_P by 3, lower k_RP and test the mode, and its accumulator
+ * lane already holds 0.  The zero block sits right before the run loop, so
+ * the switch into run mode falls through.  k_RP's step (p ? p - 1 : -2, clamped to
  * [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the carry-in of
  * s_addc from the SCC of p << kR.  The clamp to 128 never binds here: a code
  * is decoded only when p + 1 + k_R <= 32 (run paths: p + k_R <= 29), and
@@ -163,10 +165,11 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * clamp to 0 on a non-zero k = 0 symbol: p = 0 with u != 0 needs k_R >= 1,
  * i.e. k_RP >= 4.  Only the zero symbols and the run paths (where u = 0 is a
  * valid symbol, 1) keep it.  A run entry whose prefix has no one-bits
- * (the common case) has its own path (30/40): its zero count is the k-bit
- * suffix alone and `neg` advances in place (undone on the out-of-line exits
- * 37/47); prefixes with one-bits take the general path (36/46, with the
- * loop 31/41).  Refills (11/13, 18/20, 33/43, 38/48) are out of line.
+ * (the common case) has its own path in each run copy: its zero count is
+ * the k-bit suffix alone and `neg` advances in place (undone on the
+ * out-of-line exits 37/47/67/77); prefixes with one-bits take the one general
+ * path, laid out as entry (36), one-bits loop (31/35) and symbol (32) in a
+ * line.  Refills (11/13/51/55, 20, 33/43/63/73, 38) are out of line.
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window, so every check precedes any state change.  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
@@ -199,7 +202,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
-        "s_cbranch_scc0 15f\n\t"
+        "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
@@ -240,6 +243,64 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "14:\n\t"
         "s_add_u32 %[neg], %[neg], 1\n\t"
         "s_cbranch_scc1 8f\n\t"
+        "50:\n\t"
+        /* ---- k = 0 symbol ---- */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshr_b32 s85, %[kRP], 2\n\t"
+        "s_add_u32 s87, s86, s85\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s87\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
+        "s_cbranch_scc0 17f\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_max_i32 %[kP], %[kP], 0\n\t"
+        "v_mov_b32 %[vt], s89\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 51f\n\t"
+        "52:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "54:\n\t"
+        /* ---- k = 0 symbol ---- */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_add_u32 s86, s84, 1\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshr_b32 s85, %[kRP], 2\n\t"
+        "s_add_u32 s87, s86, s85\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s87\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
+        "s_cbranch_scc0 17f\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_max_i32 %[kP], %[kP], 0\n\t"
+        "v_mov_b32 %[vt], s89\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 55f\n\t"
+        "56:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
         "s_branch 10b\n\t"
         "11:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
@@ -267,19 +328,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 14b\n\t"
-        "15:\n\t"
-        "s_sub_i32 s90, s90, 2\n\t"
-        "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_add_u32 %[kP], %[kP], 3\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
-        "s_cbranch_scc1 18f\n\t"
-        "19:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc1 30f\n\t"
-        "s_branch 16b\n\t"
-        "18:\n\t"
+        "51:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
         "s_cbranch_scc1 7f\n\t"
@@ -291,7 +340,20 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
-        "s_branch 19b\n\t"
+        "s_branch 52b\n\t"
+        "55:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 56b\n\t"
         "20:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
@@ -363,7 +425,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_not_b64 s[80:81], %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
-        "s_cbranch_scc1 46f\n\t"
+        "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
@@ -396,6 +458,90 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 43f\n\t"
         "44:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc0 10b\n\t"
+        "60:\n\t"
+        /* ---- k > 0: run entry, no one-bits ---- */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_cbranch_scc1 36f\n\t"
+        "s_lshr_b32 s85, %[kP], 2\n\t"
+        "s_lshl_b64 s[80:81], %[win], 1\n\t"
+        "s_sub_u32 s88, 64, s85\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_add_u32 %[neg], %[neg], s90\n\t"
+        "s_cbranch_scc1 67f\n\t"
+        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
+        "s_not_b64 s[80:81], s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_add_u32 s86, s85, 2\n\t"
+        "s_add_u32 s86, s86, s84\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 67f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "v_mov_b32 %[vt], s88\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 63f\n\t"
+        "64:\n\t"
+        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_cbranch_scc1 8f\n\t"
+        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc0 10b\n\t"
+        "70:\n\t"
+        /* ---- k > 0: run entry, no one-bits ---- */
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_cbranch_scc1 36f\n\t"
+        "s_lshr_b32 s85, %[kP], 2\n\t"
+        "s_lshl_b64 s[80:81], %[win], 1\n\t"
+        "s_sub_u32 s88, 64, s85\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_add_u32 %[neg], %[neg], s90\n\t"
+        "s_cbranch_scc1 77f\n\t"
+        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
+        "s_not_b64 s[80:81], s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_add_u32 s86, s85, 2\n\t"
+        "s_add_u32 s86, s86, s84\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 77f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "v_mov_b32 %[vt], s88\n\t"
+        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
+        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 73f\n\t"
+        "74:\n\t"
         "s_add_u32 %[neg], %[neg], 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
@@ -433,10 +579,55 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "47:\n\t"
         "s_sub_u32 %[neg], %[neg], s90\n\t"
         "s_branch 9f\n\t"
+        "63:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 64b\n\t"
+        "73:\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cmp_eq_u32 s88, 0\n\t"
+        "s_cbranch_scc1 7f\n\t"
+        "s_mov_b32 s91, 0\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 s90, %[cur], s88\n\t"
+        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_branch 74b\n\t"
+        "67:\n\t"
+        "s_sub_u32 %[neg], %[neg], s90\n\t"
+        "s_branch 9f\n\t"
+        "77:\n\t"
+        "s_sub_u32 %[neg], %[neg], s90\n\t"
+        "s_branch 9f\n\t"
         "36:\n\t"
         "s_mov_b32 s82, 0\n\t"
         "s_mov_b32 s87, %[kP]\n\t"
-        "s_branch 31f\n\t"
+        "31:\n\t"
+        "s_cmp_gt_u32 s84, 6\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_sub_u32 s83, 0, s84\n\t"
+        "35:\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"
+        "s_lshl_b32 s89, 1, s85\n\t"
+        "s_add_u32 s82, s82, s89\n\t"
+        "s_add_u32 s87, s87, 2\n\t"
+        "s_add_u32 s83, s83, 1\n\t"
+        "s_cbranch_scc0 35b\n\t"
+        "s_lshr_b32 s85, s87, 2\n\t"
+        "s_cmp_gt_u32 s85, 24\n\t"
+        "s_cbranch_scc1 9f\n\t"
         "32:\n\t"
         "s_add_u32 s86, s84, 1\n\t"
         "s_lshl_b64 s[80:81], %[win], s86\n\t"
@@ -491,94 +682,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 39b\n\t"
-        "46:\n\t"
-        "s_mov_b32 s82, 0\n\t"
-        "s_mov_b32 s87, %[kP]\n\t"
-        "s_branch 41f\n\t"
-        "42:\n\t"
-        "s_add_u32 s86, s84, 1\n\t"
-        "s_lshl_b64 s[80:81], %[win], s86\n\t"
-        "s_sub_u32 s88, 64, s85\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_add_u32 s82, s82, s90\n\t"
-        "s_add_u32 s83, %[neg], s82\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
-        "s_add_u32 s86, s86, s85\n\t"
-        "s_not_b64 s[80:81], s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_add_u32 s86, s86, s84\n\t"
-        "s_add_u32 s86, s86, s89\n\t"
-        "s_add_u32 s86, s86, 1\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
-        "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
-        "s_sub_u32 %[kP], s87, 1\n\t"
-        "s_sub_i32 s91, s91, 2\n\t"
-        "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_mov_b32 %[neg], s83\n\t"
-        "v_mov_b32 %[vt], s88\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 48f\n\t"
-        "49:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc0 10b\n\t"
-        "s_branch 30b\n\t"
-        "48:\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
-        "s_mov_b32 s91, 0\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
-        "s_add_u32 %[wpos], %[wpos], 1\n\t"
-        "s_branch 49b\n\t"
-        "31:\n\t"
-        "s_cmp_gt_u32 s84, 6\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_sub_u32 s83, 0, s84\n\t"
-        "35:\n\t"
-        "s_lshr_b32 s85, s87, 2\n\t"
-        "s_lshl_b32 s89, 1, s85\n\t"
-        "s_add_u32 s82, s82, s89\n\t"
-        "s_add_u32 s87, s87, 2\n\t"
-        "s_add_u32 s83, s83, 1\n\t"
-        "s_cbranch_scc0 35b\n\t"
-        "s_lshr_b32 s85, s87, 2\n\t"
-        "s_cmp_gt_u32 s85, 24\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_branch 32b\n\t"
-        "41:\n\t"
-        "s_cmp_gt_u32 s84, 6\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_sub_u32 s83, 0, s84\n\t"
-        "45:\n\t"
-        "s_lshr_b32 s85, s87, 2\n\t"
-        "s_lshl_b32 s89, 1, s85\n\t"
-        "s_add_u32 s82, s82, s89\n\t"
-        "s_add_u32 s87, s87, 2\n\t"
-        "s_add_u32 s83, s83, 1\n\t"
-        "s_cbranch_scc0 45b\n\t"
-        "s_lshr_b32 s85, s87, 2\n\t"
-        "s_cmp_gt_u32 s85, 24\n\t"
-        "s_cbranch_scc1 9f\n\t"
-        "s_branch 42b\n\t"
         "7:\n\t"
         "s_add_u32 %[neg], %[neg], 1\n\t"
         "s_branch 9f\n\t"
