@@ -138,13 +138,15 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * (left aligned, zero below its nb valid bits) with nbx = nb - 33, the next
  * dword `wpos` of the chunk held one dword per lane in `cur`, k_RP, k_P and
  * neg = nrel - lim (mod 2^32; its carry out marks the end of the 64-symbol
- * chunk).  Per symbol:
+ * chunk).  Inside the loop neg lives in M0, the lane select v_writelane
+ * takes beside a scalar data operand.  Per symbol:
  *  - k = 0: Golomb-Rice code (unary prefix by one leading-ones count,
  *    remainder by shifts), k_P / k_RP adaptation (:255-276);
  *  - k > 0: up to 6 one-bits of the run prefix (each adds 2^k, k_P += 2),
  *    the run length in k bits, the zero run (only advances the symbol index),
  *    then the Golomb-Rice coded symbol + 1 and the adaptation of :240-251;
- *  - the raw Golomb value u is written into lane `nrel` of `acc`
+ *  - the raw Golomb value u is written by v_writelane into lane
+ *    (nrel - lim) mod 64 of `acc`, selected by the low bits of `neg`
  *    (Unsign2Sign is applied when the chunk is stored);
  *  - the window takes 32 more bits when fewer than 33 remain; the borrow of
  *    nbx -= length says so.
@@ -179,10 +181,9 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * instruction and then read as a v_readlane lane select needs 4 wait states
  * (here: four instructions and an s_nop 1). */
 __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint32_t& wpos, uint32_t& kRP,
-                                               uint32_t& kP, uint32_t& neg, uint32_t cur, int32_t& acc,
-                                               uint32_t vrev) {
-    int32_t vt;
+                                               uint32_t& kP, uint32_t& neg, uint32_t cur, int32_t& acc) {
     asm volatile(
+        "s_mov_b32 m0, %[neg]\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc1 30f\n\t"
         "10:\n\t"
@@ -206,13 +207,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
-        "v_mov_b32 %[vt], s89\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 11f\n\t"
         "12:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "16:\n\t"
         /* ---- k = 0 symbol ---- */
@@ -235,13 +234,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
-        "v_mov_b32 %[vt], s89\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 13f\n\t"
         "14:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "50:\n\t"
         /* ---- k = 0 symbol ---- */
@@ -264,13 +261,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
-        "v_mov_b32 %[vt], s89\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 51f\n\t"
         "52:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "54:\n\t"
         /* ---- k = 0 symbol ---- */
@@ -293,13 +288,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_max_i32 %[kP], %[kP], 0\n\t"
-        "v_mov_b32 %[vt], s89\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 55f\n\t"
         "56:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_branch 10b\n\t"
         "11:\n\t"
@@ -374,7 +367,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 20b\n\t"
         "21:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
@@ -388,7 +381,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_add_u32 %[neg], %[neg], s90\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 37f\n\t"
         "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
         "s_not_b64 s[80:81], s[92:93]\n\t"
@@ -410,13 +403,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_mov_b32 %[vt], s88\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 33f\n\t"
         "34:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
@@ -430,7 +421,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_add_u32 %[neg], %[neg], s90\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 47f\n\t"
         "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
         "s_not_b64 s[80:81], s[92:93]\n\t"
@@ -452,13 +443,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_mov_b32 %[vt], s88\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 43f\n\t"
         "44:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
@@ -472,7 +461,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_add_u32 %[neg], %[neg], s90\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 67f\n\t"
         "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
         "s_not_b64 s[80:81], s[92:93]\n\t"
@@ -494,13 +483,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_mov_b32 %[vt], s88\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 63f\n\t"
         "64:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
@@ -514,7 +501,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_add_u32 %[neg], %[neg], s90\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 77f\n\t"
         "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
         "s_not_b64 s[80:81], s[92:93]\n\t"
@@ -536,13 +523,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_mov_b32 %[vt], s88\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 73f\n\t"
         "74:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
@@ -574,10 +559,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 44b\n\t"
         "37:\n\t"
-        "s_sub_u32 %[neg], %[neg], s90\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
         "47:\n\t"
-        "s_sub_u32 %[neg], %[neg], s90\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
         "63:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
@@ -606,10 +591,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 74b\n\t"
         "67:\n\t"
-        "s_sub_u32 %[neg], %[neg], s90\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
         "77:\n\t"
-        "s_sub_u32 %[neg], %[neg], s90\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
         "36:\n\t"
         "s_mov_b32 s82, 0\n\t"
@@ -634,7 +619,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 s88, 64, s85\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
         "s_add_u32 s82, s82, s90\n\t"
-        "s_add_u32 s83, %[neg], s82\n\t"
+        "s_add_u32 s83, m0, s82\n\t"
         "s_cbranch_scc1 9f\n\t"
         "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
         "s_add_u32 s86, s86, s85\n\t"
@@ -646,6 +631,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s86, s86, 1\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 9f\n\t"
+        "s_mov_b32 m0, s83\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
@@ -657,14 +643,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[kP], s87, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_mov_b32 %[neg], s83\n\t"
-        "v_mov_b32 %[vt], s88\n\t"
-        "v_cmp_eq_u32 vcc, %[neg], %[vrev]\n\t"
-        "v_cndmask_b32 %[acc], %[acc], %[vt], vcc\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 38f\n\t"
         "39:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_cmp_gt_u32 %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
@@ -683,15 +666,16 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 39b\n\t"
         "7:\n\t"
-        "s_add_u32 %[neg], %[neg], 1\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
         "s_branch 9f\n\t"
         "8:\n\t"
         "9:\n\t"
+        "s_mov_b32 %[neg], m0\n\t"
 
         : [win] "+s"(win), [nbx] "+s"(nbx), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [neg] "+s"(neg),
-          [acc] "+v"(acc), [vt] "=&v"(vt)
-        : [cur] "v"(cur), [vrev] "v"(vrev)
-        : "vcc", "scc", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92",
+          [acc] "+v"(acc)
+        : [cur] "v"(cur)
+        : "scc", "m0", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92",
           "s93");
 }
 
@@ -702,18 +686,26 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
     br_refill(br);
     br_refill(br);
     const uint32_t lane = lane_id();
-    int32_t acc = 0; /* raw u of symbol acc_base + lane */
+    /* The raw u of symbol acc_base + i sits in lane (i - lim) mod 64, where lim
+     * is the chunk's symbol count (64 for all but a slice's last chunk): the
+     * lane that the scalar loop's v_writelane selects with the low six bits of
+     * its counter nrel - lim.  Full chunks are therefore in lane order. */
+    int32_t acc = 0;
     uint32_t acc_base = 0, nrel = 0;
     uint32_t kP = 0, kRP = 8;
     int st = PVVC_OK;
 
+    auto chunk_lim = [&](uint32_t base) {
+        const uint32_t r = total - base;
+        return r < 64u ? r : 64u;
+    };
     auto store = [&](uint32_t base, int32_t v) {
-        const uint32_t i = base + lane;
+        const uint32_t i = base + ((lane + chunk_lim(base)) & 63u);
         const int32_t s = static_cast<int32_t>(unsign2sign(static_cast<uint32_t>(v)));
         if (i < total) coef[i] = static_cast<float>(static_cast<long long>(s) * static_cast<long long>(qp));
     };
     auto emit = [&](uint32_t u) {
-        acc = (lane == nrel) ? static_cast<int32_t>(u) : acc;
+        acc = (lane == ((nrel - chunk_lim(acc_base)) & 63u)) ? static_cast<int32_t>(u) : acc;
         ++nrel;
     };
 
@@ -725,7 +717,7 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
             kP = __builtin_amdgcn_readfirstlane(kP);
             uint32_t neg = __builtin_amdgcn_readfirstlane(nrel - lim);
             int32_t nbx = __builtin_amdgcn_readfirstlane(br.nb - 33);
-            rlgr_fast_run(br.win, nbx, br.wpos, kRP, kP, neg, br.cur, acc, lane - lim);
+            rlgr_fast_run(br.win, nbx, br.wpos, kRP, kP, neg, br.cur, acc);
             nrel = neg + lim;
             br.nb = nbx + 33;
         }

@@ -9,8 +9,9 @@ run mode, L = 4, k_P from 0, k_RP from 8), so that a test can check every
 state the loop leaves against the decoder's definition.
 
 Register model: the operands %[win] (64-bit), %[nbx], %[wpos], %[kRP], %[kP],
-%[neg] (32-bit scalars); %[acc], %[vt] (per-lane vectors, read/write),
-%[cur], %[vrev] (per-lane inputs); s80-s93 scratch; vcc a 64-bit lane mask.
+%[neg] (32-bit scalars); %[acc] (per-lane vector, read/write), %[cur]
+(per-lane input); s80-s93 and m0 scratch.  Symbol i of a chunk of lim symbols sits
+in accumulator lane (i - lim) mod 64.
 """
 from __future__ import annotations
 
@@ -63,6 +64,8 @@ class Loop:
                 return s[x[2:-1]]
             if x == "vcc":
                 return vcc
+            if x == "m0":
+                return sg.get("m0", 0)
             m = re.fullmatch(r"s\[(\d+):(\d+)\]", x)
             if m:
                 lo, hi = int(m.group(1)), int(m.group(2))
@@ -74,6 +77,9 @@ class Loop:
             return int(x, 0) & M32 if not x.startswith("-") else (int(x) & M32)
 
         def wr(x, v, bits=32):
+            if x == "m0":
+                sg["m0"] = v & M32
+                return
             if x.startswith("%["):
                 name = x[2:-1]
                 s[name] = v & (M64 if name == "win" else M32)
@@ -179,6 +185,8 @@ class Loop:
                 s[a[0][2:-1]] = [a0[l] if (vcc >> l) & 1 else d[l] for l in range(64)]
             elif op == "v_readlane_b32":
                 wr(a[0], s[a[1][2:-1]][rd(a[2]) & 63])
+            elif op == "v_writelane_b32":
+                s[a[0][2:-1]][rd(a[2]) & 63] = rd(a[1]) & M32
             else:
                 raise NotImplementedError(op)
             pc = nxt
@@ -281,12 +289,11 @@ def make_state(bits: list[int], pos: int, kP: int, kRP: int, rng, nb: int, wpos:
             cur[lane] = v
             q += 32
     acc = [int(x) for x in rng.integers(0, 1 << 20, 64)]
-    for lane in range(nrel, 64):
-        acc[lane] = 0
+    for i in range(nrel, 64):
+        acc[(i - lim) & 63] = 0
     return {
         "win": win, "nbx": (nb - 33) & M32, "wpos": wpos & M32, "kRP": kRP, "kP": kP,
-        "neg": (nrel - lim) & M32, "acc": acc, "vt": [0] * 64, "cur": cur,
-        "vrev": [(l - lim) & M32 for l in range(64)],
+        "neg": (nrel - lim) & M32, "acc": acc, "cur": cur, "lim": lim,
     }
 
 
@@ -298,7 +305,7 @@ def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> st
     nb0, nb1 = s32(st0["nbx"]) + 33, s32(st1["nbx"]) + 33
     dw = (st1["wpos"] - st0["wpos"]) & M32
     consumed = nb0 + 32 * dw - nb1
-    lim = (-st0["vrev"][0]) & M32
+    lim = st0["lim"]
     nrel0, nrel1 = (st0["neg"] + lim) & M32, (st1["neg"] + lim) & M32
     if not (0 <= nrel1 <= lim) or nrel1 < nrel0 or consumed < 0:
         return f"bad exit counters nrel {nrel0}->{nrel1} lim {lim} consumed {consumed}"
@@ -318,7 +325,7 @@ def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> st
     for syms, adv, kP, kRP in events(b, kP, kRP):
         for off, val in syms:
             if n + off < 64:
-                acc[n + off] = val & M32
+                acc[(n + off - lim) & 63] = val & M32
         n += adv
         used = b.pos - pos
         if used == consumed:
