@@ -832,7 +832,7 @@ __device__ __forceinline__ int tree_structure(const uint8_t* __restrict__ geo, u
     const uint32_t lane = lane_id();
     const uint32_t h = geo[24];
     if (h == 0 || h > static_cast<uint32_t>(kMaxHeight)) return PVVC_ERR_BAD_SLICE;
-    uint32_t base = 0, cnt = 1, wmax = 0;
+    uint32_t base = 0, cnt = 1;
     if (lane == 0) {
         meta.cnt[0] = 1;
         meta.base[0] = 0;
@@ -858,12 +858,10 @@ __device__ __forceinline__ int tree_structure(const uint8_t* __restrict__ geo, u
             meta.cnt[l + 1] = cnt;
             meta.base[l + 1] = base;
         }
-        if (l + 3 < h) wmax = cnt > wmax ? cnt : wmax; /* levels stored between passes: 1 .. h-3 */
     }
     if (lane == 0) {
         meta.height = h;
         meta.internal = base;
-        meta.wmax = wmax;
     }
     if (cnt != n) return PVVC_ERR_UNMATCHED_SIZE; /* invert_raht_octree.cc:188-190 */
     return PVVC_OK;
@@ -1131,11 +1129,14 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
     uint32_t* wt = nullptr;
     if (st == PVVC_OK) {
         const uint64_t imax = a.geo_size - 25u;
-        /* LDS first; patches too large for the per-item LDS budget stream
-         * their coefficients through the global workspace */
-        coef = static_cast<float*>(ar.take(12ull * a.n));
+        /* LDS first, then the global workspace.  The first-child and weight
+         * arrays (8 B per internal node) come first: the top-down pass reads
+         * them on its chain of dependent loads (occupancy -> first child ->
+         * child weights -> Haar factors), while coefficients are independent
+         * loads at the end of that chain. */
         fc = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
         wt = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
+        coef = static_cast<float*>(ar.take(12ull * a.n));
         if (!coef || !fc || !wt) st = PVVC_ERR_CAPACITY;
     }
 
@@ -1167,12 +1168,17 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
     __syncthreads();
 
     PVVC_STAMP(item, 3);
-    /* ---- H + M: top-down inverse RAHT, centres, leaf output ---- */
+    /* ---- H + M: top-down inverse RAHT, centres, leaf output ----
+     * One pass per internal level: a thread per node inverts its transform
+     * and writes each child's YUV and centre into the next level's buffer;
+     * the leaves, the last level, are then emitted one per thread, in order. */
     const uint32_t h = (st == PVVC_OK) ? meta.height : 0u;
     float* pp = nullptr;
+    /* every level holds at most n nodes: internal nodes have children and the
+     * leaf count equals n (both checked by tree_structure) */
+    const uint32_t wl = a.n;
     if (st == PVVC_OK && h > 1) {
-        const uint32_t wmax = meta.wmax > 0 ? meta.wmax : 1u;
-        pp = static_cast<float*>(ar.take(48ull * wmax));
+        pp = static_cast<float*>(ar.take(48ull * wl));
         if (!pp) st = PVVC_ERR_CAPACITY;
     }
     if (st == PVVC_OK) {
@@ -1191,21 +1197,16 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
             if (tid == 0) emit_leaf(0, hdr[0], hdr[1], hdr[2], coef[n - 1], coef[2 * n - 1], coef[3 * n - 1], a, ref, rs,
                                     predict, sink);
         } else {
-            const uint32_t wmax = meta.wmax > 0 ? meta.wmax : 1u;
             float* bufA = pp;
-            float* bufB = pp + 6 * wmax;
+            float* bufB = pp + 6 * wl;
             float rx = hdr[3], ry = hdr[4], rz = hdr[5];
             for (uint32_t l = 0; l + 1 < h; ++l) {
                 const uint32_t cnt = meta.cnt[l], base = meta.base[l];
-                const uint32_t cbase = meta.base[l + 1], ccnt = meta.cnt[l + 1];
-                const bool leaf_level = (l + 2 == h); /* children are leaves (h == 2) */
-                const bool fused = (l + 3 == h);      /* children are the last internal level:
-                                                         decode them and their leaves here */
+                const uint32_t cbase = meta.base[l + 1];
+                const bool leaf_level = (l + 2 == h); /* children are leaves (weight 1) */
                 /* SubSpaceCenter: child range = range / 2, offset = child range / 2 */
                 const float sx = rx / 2.0f, sy = ry / 2.0f, sz = rz / 2.0f;
                 const float ox = sx / 2.0f, oy = sy / 2.0f, oz = sz / 2.0f;
-                const float sx2 = sx / 2.0f, sy2 = sy / 2.0f, sz2 = sz / 2.0f;
-                const float ox2 = sx2 / 2.0f, oy2 = sy2 / 2.0f, oz2 = sz2 / 2.0f;
                 float* cur = (l & 1) ? bufB : bufA;
                 float* nxt = (l & 1) ? bufA : bufB;
                 for (uint32_t j = tid; j < cnt; j += NT) {
@@ -1221,11 +1222,11 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
                         cz = hdr[2];
                     } else {
                         dY = cur[j];
-                        dU = cur[wmax + j];
-                        dV = cur[2 * wmax + j];
-                        cx = cur[3 * wmax + j];
-                        cy = cur[4 * wmax + j];
-                        cz = cur[5 * wmax + j];
+                        dU = cur[wl + j];
+                        dV = cur[2 * wl + j];
+                        cx = cur[3 * wl + j];
+                        cy = cur[4 * wl + j];
+                        cz = cur[5 * wl + j];
                     }
                     uint32_t cw[8];
                     uint32_t rank = 0;
@@ -1243,46 +1244,24 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
                         if (!((occ >> (7 - i)) & 1u)) continue;
                         const uint32_t c = f + k;
                         ++k;
-                        const float ccx = (i & 4) ? cx - ox : cx + ox;
-                        const float ccy = (i & 2) ? cy - oy : cy + oy;
-                        const float ccz = (i & 1) ? cz - oz : cz + oz;
-                        if (leaf_level) {
-                            emit_leaf(c, ccx, ccy, ccz, oY[i], oU[i], oV[i], a, ref, rs, predict, sink);
-                        } else if (fused) {
-                            /* child c on level h-2: its children are leaves of weight 1 */
-                            const uint32_t cocc = a.geo[25 + cbase + c];
-                            const uint32_t cf = fc[cbase + c];
-                            uint32_t lw[8];
-#pragma unroll
-                            for (int q = 0; q < 8; ++q) lw[q] = (cocc >> (7 - q)) & 1u;
-                            float lY[8], lU[8], lV[8];
-                            node_inverse(lw, oY[i], oU[i], oV[i], coef, n, ccnt + cf - c, lY, lU, lV);
-                            uint32_t m = 0;
-#pragma unroll
-                            for (int q = 0; q < 8; ++q) {
-                                if (!lw[q]) continue;
-                                const float lx = (q & 4) ? ccx - ox2 : ccx + ox2;
-                                const float ly = (q & 2) ? ccy - oy2 : ccy + oy2;
-                                const float lz = (q & 1) ? ccz - oz2 : ccz + oz2;
-                                emit_leaf(cf + m, lx, ly, lz, lY[q], lU[q], lV[q], a, ref, rs, predict, sink);
-                                ++m;
-                            }
-                        } else {
-                            nxt[c] = oY[i];
-                            nxt[wmax + c] = oU[i];
-                            nxt[2 * wmax + c] = oV[i];
-                            nxt[3 * wmax + c] = ccx;
-                            nxt[4 * wmax + c] = ccy;
-                            nxt[5 * wmax + c] = ccz;
-                        }
+                        nxt[c] = oY[i];
+                        nxt[wl + c] = oU[i];
+                        nxt[2 * wl + c] = oV[i];
+                        nxt[3 * wl + c] = (i & 4) ? cx - ox : cx + ox;
+                        nxt[4 * wl + c] = (i & 2) ? cy - oy : cy + oy;
+                        nxt[5 * wl + c] = (i & 1) ? cz - oz : cz + oz;
                     }
                 }
                 __syncthreads();
-                if (fused) break;
                 rx = sx;
                 ry = sy;
                 rz = sz;
             }
+            /* the leaves were written by the pass of level h - 2 */
+            const float* lv = ((h - 2) & 1) ? bufA : bufB;
+            for (uint32_t c = tid; c < n; c += NT)
+                emit_leaf(c, lv[3 * wl + c], lv[4 * wl + c], lv[5 * wl + c], lv[c], lv[wl + c], lv[2 * wl + c], a, ref, rs,
+                          predict, sink);
         }
     }
 

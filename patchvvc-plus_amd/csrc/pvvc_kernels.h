@@ -119,7 +119,6 @@ struct LevelMeta {
     uint32_t base[kMaxHeight + 1];  /* first internal-node index of the level */
     uint32_t height;
     uint32_t internal;              /* internal node count */
-    uint32_t wmax;                  /* widest internal level below the root */
     int32_t tree_status;
     int32_t rlgr_status;
     uint32_t pad[3];
