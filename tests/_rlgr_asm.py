@@ -20,6 +20,9 @@ from pathlib import Path
 
 M32 = (1 << 32) - 1
 M64 = (1 << 64) - 1
+# The loop keeps nbx = nb - NB_BIAS for a window of nb valid bits and is
+# entered with NB_MIN <= nb <= NB_MAX.
+NB_BIAS, NB_MIN, NB_MAX = 33, 33, 64
 HIP_SOURCE = Path(__file__).resolve().parents[1] / "patchvvc-plus_amd" / "csrc" / "pvvc_decode.hip"
 
 
@@ -292,7 +295,7 @@ def make_state(bits: list[int], pos: int, kP: int, kRP: int, rng, nb: int, wpos:
     for i in range(nrel, 64):
         acc[(i - lim) & 63] = 0
     return {
-        "win": win, "nbx": (nb - 33) & M32, "wpos": wpos & M32, "kRP": kRP, "kP": kP,
+        "win": win, "nbx": (nb - NB_BIAS) & M32, "wpos": wpos & M32, "kRP": kRP, "kP": kP,
         "neg": (nrel - lim) & M32, "acc": acc, "cur": cur, "lim": lim,
     }
 
@@ -302,7 +305,7 @@ def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> st
     whole number of events from st0; otherwise a description."""
     def s32(v):
         return v - (1 << 32) if v >> 31 else v
-    nb0, nb1 = s32(st0["nbx"]) + 33, s32(st1["nbx"]) + 33
+    nb0, nb1 = s32(st0["nbx"]) + NB_BIAS, s32(st1["nbx"]) + NB_BIAS
     dw = (st1["wpos"] - st0["wpos"]) & M32
     consumed = nb0 + 32 * dw - nb1
     lim = st0["lim"]

@@ -27,7 +27,7 @@ def _cases(oracle, seed, streams_per_kind, states_per_stream):
             bnd = R.decode_boundaries(bits, L)
             for _ in range(states_per_stream):
                 pos, kP, kRP, _n = bnd[int(rng.integers(0, len(bnd)))]
-                nb = int(rng.integers(33, 65))
+                nb = int(rng.integers(R.NB_MIN, R.NB_MAX + 1))
                 wpos = int(rng.integers(0, 1 << 20))
                 lim = int(rng.integers(1, 65))
                 nrel = int(rng.integers(0, lim))
