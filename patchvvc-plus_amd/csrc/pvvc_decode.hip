@@ -912,9 +912,11 @@ __device__ __forceinline__ void write_points_rgb(Points* pt, float r, float g, f
     }
 }
 
-__device__ __forceinline__ void emit_leaf(uint32_t c, float cx, float cy, float cz, float Y, float U, float V,
-                                          const SliceArgs& a, float* ref, uint32_t ref_stride, bool predict,
-                                          const Sink& sink) {
+/* Leaf output with the reference colour (rY, rU, rV) already loaded by the
+ * caller for a predictive slice that is not deferred (ignored otherwise). */
+__device__ __forceinline__ void emit_leaf_ref(uint32_t c, float cx, float cy, float cz, float Y, float U, float V,
+                                              float rY, float rU, float rV, const SliceArgs& a, float* ref,
+                                              uint32_t ref_stride, bool predict, const Sink& sink) {
     const bool deferred = predict && a.stage != nullptr;
     if (deferred) { /* reference added after the launch (finish_kernel) */
         float* sp = a.stage + 3 * (static_cast<uint64_t>(a.stage_off) + c);
@@ -922,9 +924,9 @@ __device__ __forceinline__ void emit_leaf(uint32_t c, float cx, float cy, float 
         sp[1] = U;
         sp[2] = V;
     } else if (predict) { /* invert_raht_octree.cc:304-308 */
-        Y = Y + ref[c];
-        U = U + ref[ref_stride + c];
-        V = V + ref[2 * ref_stride + c];
+        Y = Y + rY;
+        U = U + rU;
+        V = V + rV;
     } else { /* :309-311 */
         ref[c] = Y;
         ref[ref_stride + c] = U;
@@ -956,6 +958,16 @@ __device__ __forceinline__ void emit_leaf(uint32_t c, float cx, float cy, float 
         if (deferred) return;
         write_points_rgb(pt, r, g, b, a.n, sink.white_small);
     }
+}
+
+__device__ __forceinline__ void emit_leaf(uint32_t c, float cx, float cy, float cz, float Y, float U, float V,
+                                          const SliceArgs& a, float* ref, uint32_t ref_stride, bool predict,
+                                          const Sink& sink) {
+    const bool load_ref = predict && a.stage == nullptr;
+    const float rY = load_ref ? ref[c] : 0.0f;
+    const float rU = load_ref ? ref[ref_stride + c] : 0.0f;
+    const float rV = load_ref ? ref[2 * ref_stride + c] : 0.0f;
+    emit_leaf_ref(c, cx, cy, cz, Y, U, V, rY, rU, rV, a, ref, ref_stride, predict, sink);
 }
 
 /* One inverse Haar butterfly pair (octree_base.cc:186-206) on channel values;
@@ -1257,11 +1269,31 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
                 ry = sy;
                 rz = sz;
             }
-            /* the leaves were written by the pass of level h - 2 */
+            /* the leaves were written by the pass of level h - 2; each thread
+             * issues the loads of four leaves (and their reference colours)
+             * before any of their stores */
             const float* lv = ((h - 2) & 1) ? bufA : bufB;
-            for (uint32_t c = tid; c < n; c += NT)
-                emit_leaf(c, lv[3 * wl + c], lv[4 * wl + c], lv[5 * wl + c], lv[c], lv[wl + c], lv[2 * wl + c], a, ref, rs,
-                          predict, sink);
+            const bool load_ref = predict && a.stage == nullptr;
+            for (uint32_t c0 = tid; c0 < n; c0 += 4 * NT) {
+                float v[4][6], r[4][3];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t c = c0 + q * NT;
+                    if (c < n) {
+#pragma unroll
+                        for (int e = 0; e < 6; ++e) v[q][e] = lv[e * wl + c];
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) r[q][e] = load_ref ? ref[e * rs + c] : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t c = c0 + q * NT;
+                    if (c < n)
+                        emit_leaf_ref(c, v[q][3], v[q][4], v[q][5], v[q][0], v[q][1], v[q][2], r[q][0], r[q][1], r[q][2],
+                                      a, ref, rs, predict, sink);
+                }
+            }
         }
     }
 
