@@ -88,7 +88,7 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
 
 /* Launch shape of launch_cudaProcess: wavefronts per slice (1, 2, 4) and the
  * LDS workspace per slice in bytes (arrays that do not fit use the state
- * slot's global workspace).  Default: 2 wavefronts, 32 KiB. */
+ * slot's global workspace).  Default: 4 wavefronts, 32 KiB. */
 int pvvc_set_dropin_tuning(int waves, int lds_bytes);
 /* Stream used by launch_cudaProcess and the native calls (a hipStream_t). */
 int pvvc_set_stream(void* stream);

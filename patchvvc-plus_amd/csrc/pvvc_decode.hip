@@ -1892,10 +1892,11 @@ struct TuningState {
     int wgs_per_cu = 8;
     int queue_chunk = 1;
 };
-/* 2 waves per slice: the second builds the octree while the first decodes
- * RLGR; on a 10-bit 816k-point frame of 820 slices this measured 0.726 ms
- * against 0.784 ms for 1 wave and 0.814 ms for 4 */
-TuningState g_dropin_tuning{2, 32 * 1024, 8, 1};
+/* 4 waves per slice: the second builds the octree while the first decodes
+ * RLGR, and all four share the top-down passes and the leaf output; on a
+ * 10-bit 816k-point frame of 820 slices this measured 0.521 ms against
+ * 0.533 ms for 2 waves and 0.618 ms for 1 (16-32 KiB of LDS alike) */
+TuningState g_dropin_tuning{4, 32 * 1024, 8, 1};
 
 int clamp_lds(int want) {
     int mx = devinfo().lds_max > 0 ? devinfo().lds_max : 65536;

@@ -56,7 +56,7 @@ def test_gpu_every_last_chunk_count(pvvc, oracle, tmp_path):
         pts = pvvc.decode_dropin(f, pvvc.State(S, mx))
         np.testing.assert_array_equal(np.stack([pts["x"], pts["y"], pts["z"]], 1), ref["xyz"])
         np.testing.assert_array_equal(np.stack([pts["r"], pts["g"], pts["b"]], 1), ref["rgbf"] / np.float32(255))
-    assert pvvc.hip_lib().pvvc_set_dropin_tuning(2, 32 * 1024) == 0  # the default
+    assert pvvc.hip_lib().pvvc_set_dropin_tuning(4, 32 * 1024) == 0  # the default
     for mode in (pvvc.MODE_PER_FRAME, pvvc.MODE_PERSISTENT, pvvc.MODE_GRAPH):
         _say("residues native mode", mode)
         dec = pvvc.Decoder(S, mx)

@@ -50,7 +50,7 @@ def test_dropin_c0_matches_oracle(pvvc, c0_files, c0_oracle, waves):
         np.testing.assert_array_equal(xyz, ref["xyz"])
         rgbf = np.stack([pts["r"], pts["g"], pts["b"]], 1)
         np.testing.assert_array_equal(rgbf, ref["rgbf"] / np.float32(255))
-    assert pvvc.hip_lib().pvvc_set_dropin_tuning(2, 32 * 1024) == 0  # the default
+    assert pvvc.hip_lib().pvvc_set_dropin_tuning(4, 32 * 1024) == 0  # the default
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2], ids=["per_frame", "persistent", "graph"])

@@ -120,7 +120,7 @@ def test_gpu_rlgr_stress_matches_oracle(pvvc, oracle, tmp_path):
         for s in range(n_patches):
             assert st.status(s) == 0, syms[s][0]
         np.testing.assert_array_equal(np.stack([pts["r"], pts["g"], pts["b"]], 1), want)
-    assert pvvc.hip_lib().pvvc_set_dropin_tuning(2, 32 * 1024) == 0  # the default
+    assert pvvc.hip_lib().pvvc_set_dropin_tuning(4, 32 * 1024) == 0  # the default
     _say("rlgr stress persistent")
     dec = pvvc.Decoder(n_patches, max_pts)
     b = dec.batch([f])
