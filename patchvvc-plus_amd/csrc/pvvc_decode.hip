@@ -2939,7 +2939,8 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     }
     const uint64_t total = at;
     if (!enc_grow_dev(total) || !enc_grow_pinned(g_enc.pin, g_enc.pin_cap, in_end)) return PVVC_ERR_HIP;
-    if (!g_enc.e0 && !(hip_ok(hipEventCreate(&g_enc.e0)) && hip_ok(hipEventCreate(&g_enc.e1)))) return PVVC_ERR_HIP;
+    if (!g_enc.e0 && !hip_ok(hipEventCreate(&g_enc.e0))) return PVVC_ERR_HIP;
+    if (!g_enc.e1 && !hip_ok(hipEventCreate(&g_enc.e1))) return PVVC_ERR_HIP;
     uint8_t* const dev = g_enc.dev;
     uint8_t* const h = g_enc.pin;
     {
@@ -3021,7 +3022,10 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     uint8_t* spill = nullptr; /* only when the outputs outgrow the input region */
     if (good && packed_bytes) {
         uint8_t* packed = dev + in_begin;
-        if (packed_bytes > in_bytes) good = hip_ok(hipMalloc(&spill, packed_bytes)), packed = spill;
+        if (packed_bytes > in_bytes) {
+            good = hip_ok(hipMalloc(&spill, packed_bytes));
+            packed = spill;
+        }
         good = good && enc_grow_pinned(g_enc.pin_out, g_enc.pin_out_cap, packed_bytes) &&
                hip_ok(hipMemcpyAsync(dev + o_at, pos.data(), 8ull * n_jobs, hipMemcpyHostToDevice, g_stream));
         if (good) {
