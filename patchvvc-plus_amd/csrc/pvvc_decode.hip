@@ -1025,7 +1025,12 @@ __global__ void raht_table_kernel() {
     g_raht_tab[i] = (w0 && w1) ? haar_factors(w0, w1) : make_float2(0.0f, 0.0f);
 }
 
-__device__ __forceinline__ void node_inverse(const uint32_t cw[8], float dY, float dU, float dV, const float* coef,
+/* Haar factors of weight pairs below kSmallTab, copied per workgroup into LDS:
+ * every pair of a node whose children are leaves has weights of at most 4. */
+constexpr uint32_t kSmallTab = 8;
+
+__device__ __forceinline__ void node_inverse(const float2* small_tab, const uint32_t cw[8], float dY, float dU, float dV,
+                                             const float* coef,
                                              uint32_t n, uint32_t r, float oY[8], float oU[8], float oV[8]) {
     uint32_t w[16];
 #pragma unroll
@@ -1048,7 +1053,9 @@ __device__ __forceinline__ void node_inverse(const uint32_t cw[8], float dY, flo
         if (w0 != 0 && w1 != 0) {
             const float hY = coef[n - 1 - r], hU = coef[2 * n - 1 - r], hV = coef[3 * n - 1 - r];
             ++r;
-            const float2 t = (w0 < kRahtTab && w1 < kRahtTab) ? g_raht_tab[w0 * kRahtTab + w1] : haar_factors(w0, w1);
+            const float2 t = (w0 < kSmallTab && w1 < kSmallTab) ? small_tab[w0 * kSmallTab + w1]
+                             : (w0 < kRahtTab && w1 < kRahtTab)   ? g_raht_tab[w0 * kRahtTab + w1]
+                                                                  : haar_factors(w0, w1);
             const float ta = t.x, tc = t.y;
             cY[ia] = cY[idx] * ta + hY * (-tc);
             cY[ib] = cY[idx] * tc + hY * ta;
@@ -1180,6 +1187,10 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
     __syncthreads();
 
     PVVC_STAMP(item, 3);
+    __shared__ float2 s_tab[kSmallTab * kSmallTab];
+    for (uint32_t i = tid; i < kSmallTab * kSmallTab; i += NT)
+        s_tab[i] = g_raht_tab[(i / kSmallTab) * kRahtTab + i % kSmallTab];
+    __syncthreads();
     /* ---- H + M: top-down inverse RAHT, centres, leaf output ----
      * One pass per internal level: a thread per node inverts its transform
      * and writes each child's YUV and centre into the next level's buffer;
@@ -1249,7 +1260,7 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
                         rank += has ? 1u : 0u;
                     }
                     float oY[8], oU[8], oV[8];
-                    node_inverse(cw, dY, dU, dV, coef, n, cnt + f - j, oY, oU, oV);
+                    node_inverse(s_tab, cw, dY, dU, dV, coef, n, cnt + f - j, oY, oU, oV);
                     uint32_t k = 0;
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
@@ -1900,7 +1911,7 @@ TuningState g_dropin_tuning{4, 32 * 1024, 8, 1};
 
 int clamp_lds(int want) {
     int mx = devinfo().lds_max > 0 ? devinfo().lds_max : 65536;
-    int reserve = static_cast<int>(sizeof(LevelMeta)) + 64;
+    int reserve = static_cast<int>(sizeof(LevelMeta) + sizeof(float2) * kSmallTab * kSmallTab) + 64;
     want = std::max(0, std::min(want, mx - reserve));
     return want & ~15;
 }
