@@ -36,7 +36,8 @@ enum pvvc_status {
     PVVC_ERR_CAPACITY = 7,         /* patch larger than its state slot / workspace */
     PVVC_ERR_INVALID_ARG = 8,
     PVVC_ERR_HIP = 9,              /* a HIP runtime call failed */
-    PVVC_ERR_TIMEOUT = 10,         /* waited too long for the intra slice of the same patch */
+    PVVC_ERR_TIMEOUT = 10,         /* reserved, never returned: predictive slices never wait in a
+                                      kernel (a finish pass adds their reference colours) */
     PVVC_ERR_IO = 11,              /* file error (see pvvc_load_frame for the reference codes) */
     PVVC_ERR_BUSY = 12             /* serving: no slot / memory to admit a stream */
 };
