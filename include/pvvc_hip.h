@@ -175,7 +175,9 @@ typedef struct pvvc_tuning {
     int wgs_per_cu;        /* persistent workgroups per CU */
     int queue_chunk;       /* items taken per queue pop */
     int priority_items;    /* the first N items of the length-sorted queue run at the highest
-                              wave priority (s_setprio 3); -1 = none */
+                              wave priority (s_setprio 3), the rest of the queue in thirds at
+                              2, 1 and 0; -1 = all at 0; default: N = one item per persistent
+                              workgroup (the items that start together) */
 } pvvc_tuning;
 
 int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_patch);

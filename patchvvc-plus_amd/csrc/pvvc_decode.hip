@@ -1437,12 +1437,21 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
             Arena ar{g_smem, lds_bytes, 0, ws, slab_bytes, 0};
             SliceArgs a = item_args(it);
             a.stage = stage;
-            /* the longest slices (queue head) win issue arbitration on their CU,
-             * so that the launch is not stretched by them running contended */
-            const bool hi = q < prio_items;
-            if (hi) __builtin_amdgcn_s_setprio(3);
+            /* wave priority falls with queue rank: the first prio_items
+             * (longest) slices at 3, the rest of the queue in thirds at 2, 1
+             * and 0, so that issue arbitration on a CU favours the slice with
+             * more work left over the oldest wavefront */
+            const uint32_t rest = count > prio_items ? count - prio_items : 0u;
+            const int lvl = prio_items == 0u                 ? 0
+                            : q < prio_items                 ? 3
+                            : q < prio_items + rest / 3      ? 2
+                            : q < prio_items + 2 * (rest / 3) ? 1
+                                                             : 0;
+            if (lvl == 3) __builtin_amdgcn_s_setprio(3);
+            else if (lvl == 2) __builtin_amdgcn_s_setprio(2);
+            else if (lvl == 1) __builtin_amdgcn_s_setprio(1);
             int st = decode_slice<WAVES>(a, slot, ar, sink, meta, launch_id, true, i);
-            if (hi) __builtin_amdgcn_s_setprio(0);
+            if (lvl) __builtin_amdgcn_s_setprio(0);
             if (threadIdx.x == 0) status[i] = st;
             __syncthreads();
         }
@@ -2285,7 +2294,15 @@ void launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, cons
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(t.lds_bytes));
     const uint32_t grid = persistent_grid(t, sg.count);
     const uint32_t chunk = static_cast<uint32_t>(std::max(1, t.queue_chunk));
-    const uint32_t prio = static_cast<uint32_t>(std::max(0, t.priority_items));
+    /* default: the first wave of the queue, one item per workgroup, at the
+     * top priority.  Those items start together; the scalar-issue arbiter
+     * otherwise serves the oldest wavefronts first, so a workgroup that pops
+     * a second item keeps starving a younger one still on its first (larger)
+     * item, and that item ends the launch (c1 GoP: 0.950 -> 0.908 ms; with
+     * the graded levels of the rest of the queue 0.899 ms) */
+    const uint32_t prio = t.priority_items < 0 ? 0u
+                          : t.priority_items == 0 ? grid
+                                                  : static_cast<uint32_t>(t.priority_items);
     if (t.waves_per_patch == 2) launch_persistent_w<2>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
     else if (t.waves_per_patch == 4) launch_persistent_w<4>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
     else launch_persistent_w<1>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
