@@ -1519,6 +1519,7 @@ struct EncJobDev {
     uint32_t n;
     uint32_t out_cap;
     uint32_t qp;
+    uint32_t idx;          /* the job's index in the caller's table */
 };
 
 /* HierarchicalTransform (octree_base.cc:65-82) of one node: child DCs in
@@ -1811,28 +1812,38 @@ __device__ __forceinline__ int64_t encode_job(const EncJobDev& jb, Arena ar, Lev
     return bytes < 0 ? -PVVC_ERR_CAPACITY : bytes;
 }
 
-/* Packs each job's bytes at its offset of a contiguous buffer. */
+/* Packs each job's bytes at its offset of a contiguous buffer (size and at
+ * are indexed by the caller's job index, jobs by rank). */
 __global__ __launch_bounds__(256) void gather_kernel(const EncJobDev* __restrict__ jobs, const int64_t* __restrict__ size,
                                                      const uint64_t* __restrict__ at, uint32_t count, uint8_t* packed) {
-    const uint32_t i = blockIdx.x;
-    if (i >= count) return;
+    if (blockIdx.x >= count) return;
+    const EncJobDev& jb = jobs[blockIdx.x];
+    const uint32_t i = jb.idx;
     const int64_t n = size[i];
     if (n <= 0) return;
-    const uint8_t* src = jobs[i].out;
+    const uint8_t* src = jb.out;
     uint8_t* dst = packed + at[i];
     for (int64_t b = threadIdx.x; b < n; b += blockDim.x) dst[b] = src[b];
 }
 
-/* One wavefront per colour stream. */
+/* One wavefront per colour stream.  Jobs come longest first; workgroups
+ * start in index order.  Wave priority falls with rank as in the persistent
+ * decoder: the first `wave1` jobs (one per resident workgroup) at 3, the rest
+ * in thirds at 2, 1 and 0. */
 __global__ __launch_bounds__(64) void encode_kernel(const EncJobDev* __restrict__ jobs, uint32_t count,
-                                                    uint32_t lds_bytes, int64_t* __restrict__ result) {
+                                                    uint32_t lds_bytes, int64_t* __restrict__ result,
+                                                    uint32_t wave1) {
     __shared__ LevelMeta meta;
-    const uint32_t i = blockIdx.x;
-    if (i >= count) return;
-    const EncJobDev jb = jobs[i];
+    const uint32_t q = blockIdx.x;
+    if (q >= count) return;
+    const uint32_t rest = count > wave1 ? count - wave1 : 0u;
+    if (q < wave1) __builtin_amdgcn_s_setprio(3);
+    else if (q < wave1 + rest / 3) __builtin_amdgcn_s_setprio(2);
+    else if (q < wave1 + 2 * (rest / 3)) __builtin_amdgcn_s_setprio(1);
+    const EncJobDev jb = jobs[q];
     Arena ar{g_smem, lds_bytes, 0, jb.ws, jb.ws_cap, 0};
     const int64_t r = encode_job(jb, ar, meta);
-    if (threadIdx.x == 0) result[i] = r;
+    if (threadIdx.x == 0) result[jb.idx] = r;
 }
 
 }  // namespace pvvc
@@ -2924,7 +2935,7 @@ int host_threads(int tasks) {
 extern "C" {
 
 /* Steps of one call:
- *  1. layout: device image [EncJobDev x n][results][offsets][inputs: geo |
+ *  1. layout: device image [EncJobDev x n, longest first][results][offsets][inputs: geo |
  *     yuv per job][outputs][workspaces]; the pinned buffer holds the first
  *     four parts at the same offsets;
  *  2. host threads copy the jobs' inputs into the pinned buffer, in chunks of
@@ -2970,8 +2981,13 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     uint8_t* const dev = g_enc.dev;
     uint8_t* const h = g_enc.pin;
     {
+        /* longest jobs first, so that the last workgroups to start are short */
+        std::vector<int> order(n_jobs);
+        for (int i = 0; i < n_jobs; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].n > jobs[y].n; });
         EncJobDev* jd = reinterpret_cast<EncJobDev*>(h + o_jobs);
-        for (int i = 0; i < n_jobs; ++i) {
+        for (int r = 0; r < n_jobs; ++r) {
+            const int i = order[r];
             EncJobDev d;
             d.geo = dev + o_geo[i];
             d.yuv = reinterpret_cast<const float*>(dev + o_yuv[i]);
@@ -2982,7 +2998,8 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
             d.n = jobs[i].n;
             d.out_cap = static_cast<uint32_t>(out_cap[i]);
             d.qp = jobs[i].qp;
-            jd[i] = d;
+            d.idx = static_cast<uint32_t>(i);
+            jd[r] = d;
         }
     }
     bool good = hip_ok(hipMemcpyAsync(dev + o_jobs, h + o_jobs, o_res, hipMemcpyHostToDevice, g_stream));
@@ -3028,9 +3045,10 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     std::vector<int64_t> res(n_jobs, 0);
     if (good) {
         hipEventRecord(g_enc.e0, g_stream);
+        const uint32_t wave1 = static_cast<uint32_t>(std::max(1, devinfo().cu) * 16);
         hipLaunchKernelGGL(encode_kernel, dim3(n_jobs), dim3(64), lds, g_stream,
                            reinterpret_cast<const EncJobDev*>(dev + o_jobs), static_cast<uint32_t>(n_jobs), lds,
-                           reinterpret_cast<int64_t*>(dev + o_res));
+                           reinterpret_cast<int64_t*>(dev + o_res), wave1);
         hipEventRecord(g_enc.e1, g_stream);
         good = hip_ok(hipGetLastError()) &&
                hip_ok(hipMemcpyAsync(res.data(), dev + o_res, 8ull * n_jobs, hipMemcpyDeviceToHost, g_stream)) &&
