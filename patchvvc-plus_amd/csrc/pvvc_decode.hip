@@ -1456,9 +1456,20 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
             __syncthreads();
         }
     }
+    /* queue[1] counts workgroups out; the last one re-arms both counters for
+     * the next launch: every other workgroup has made its final pop by then
+     * (its exit increment follows the pop that returned past `count`) */
+    if (threadIdx.x == 0 && atomicAdd(queue + 1, 1u) == gridDim.x - 1) {
+        atomicExch(queue, 0u);
+        atomicExch(queue + 1, 0u);
+    }
 }
 
-__global__ void reset_counter(uint32_t* q) { *q = 0; }
+/* Zeroes a queue's counters once, after allocation. */
+__global__ void reset_counter(uint32_t* q) {
+    q[0] = 0;
+    q[1] = 0;
+}
 
 /* Finish pass of deferred predictive items: the intra slice of the same
  * patch decoded in the launch before this one; add its colours
@@ -2230,7 +2241,6 @@ void launch_frame(const pvvc_batch* b, int f, const Sink& sink, uint32_t lds, hi
 template <int W>
 void launch_persistent_w(const BatchDev& b, const Segment& sg, const Sink& sink, uint32_t lds, uint32_t grid,
                          uint32_t chunk, uint32_t prio, const LaunchRes& r, uint32_t launch_id, hipStream_t s) {
-    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, r.queue);
     auto ev = g_ktiming.next();
     hipEventRecord(ev.first, s);
     hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b.items, b.order + sg.begin, sg.count,
@@ -2621,7 +2631,8 @@ int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_
         delete d;
         return PVVC_ERR_HIP;
     }
-    if (!hip_ok(hipMalloc(&d->queue, 64))) {
+    if (!hip_ok(hipMalloc(&d->queue, 64)) || !hip_ok(hipMemset(d->queue, 0, 64))) {
+        if (d->queue) hipFree(d->queue);
         state_destroy(d->st);
         delete d;
         return PVVC_ERR_HIP;
