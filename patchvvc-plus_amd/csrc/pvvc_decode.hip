@@ -373,9 +373,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 10b\n\t"
         "30:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
@@ -413,9 +411,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 10b\n\t"
         "40:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
@@ -453,9 +449,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 10b\n\t"
         "60:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
@@ -493,9 +487,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 10b\n\t"
         "70:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_lshl_b64 s[80:81], %[win], 1\n\t"
@@ -597,6 +589,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
         "36:\n\t"
+        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_mov_b32 s82, 0\n\t"
         "s_mov_b32 s87, %[kP]\n\t"
         "31:\n\t"

@@ -170,6 +170,8 @@ class Loop:
                 v = (rd(a[1]) | rd(a[2])) & M64
                 wr(a[0], v)
                 scc = int(v != 0)
+            elif op == "s_bitcmp1_b64":
+                scc = (rd(a[0]) >> (rd(a[1]) & 63)) & 1
             elif op.startswith("s_cmp_"):
                 x, y = rd(a[0]) & M32, rd(a[1]) & M32
                 scc = int({"s_cmp_gt_u32": x > y, "s_cmp_eq_u32": x == y, "s_cmp_lg_u32": x != y,
