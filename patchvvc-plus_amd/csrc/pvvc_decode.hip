@@ -171,7 +171,13 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * the k-bit suffix alone and `neg` advances in place (undone on the
  * out-of-line exits 37/47/67/77); prefixes with one-bits take the one general
  * path, laid out as entry (36), one-bits loop (31/35) and symbol (32) in a
- * line.  Refills (11/13/51/55, 20, 33/43/63/73, 38) are out of line.
+ * line; a one-bit test of the window's top bit chooses between them.  The
+ * leading-ones count of a Golomb code is s_flbit of s_orn2(1, window): that
+ * value is never 0, so the count stays below 64 and s_orn2 sets SCC, which
+ * s_flbit leaves for the s_addc that adds the code's terminating zero.  A
+ * run's zero count is read straight from the window (its top bit is the
+ * prefix's terminating zero).  Refills (11/13/51/55, 20, 33/43/63/73, 38)
+ * are out of line.
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window, so every check precedes any state change.  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
@@ -188,12 +194,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 30f\n\t"
         "10:\n\t"
         /* ---- k = 0 symbol ---- */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_add_u32 s86, s84, 1\n\t"
-        "s_cbranch_scc1 9f\n\t"
         "s_lshr_b32 s85, %[kRP], 2\n\t"
-        "s_add_u32 s87, s86, s85\n\t"
+        "s_orn2_b64 s[80:81], 1, %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s87, s84, s85\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 9f\n\t"
         "s_lshl_b64 s[80:81], %[win], s84\n\t"
@@ -215,12 +219,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 8f\n\t"
         "16:\n\t"
         /* ---- k = 0 symbol ---- */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_add_u32 s86, s84, 1\n\t"
-        "s_cbranch_scc1 9f\n\t"
         "s_lshr_b32 s85, %[kRP], 2\n\t"
-        "s_add_u32 s87, s86, s85\n\t"
+        "s_orn2_b64 s[80:81], 1, %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s87, s84, s85\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 9f\n\t"
         "s_lshl_b64 s[80:81], %[win], s84\n\t"
@@ -242,12 +244,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 8f\n\t"
         "50:\n\t"
         /* ---- k = 0 symbol ---- */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_add_u32 s86, s84, 1\n\t"
-        "s_cbranch_scc1 9f\n\t"
         "s_lshr_b32 s85, %[kRP], 2\n\t"
-        "s_add_u32 s87, s86, s85\n\t"
+        "s_orn2_b64 s[80:81], 1, %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s87, s84, s85\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 9f\n\t"
         "s_lshl_b64 s[80:81], %[win], s84\n\t"
@@ -269,12 +269,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 8f\n\t"
         "54:\n\t"
         /* ---- k = 0 symbol ---- */
-        "s_not_b64 s[80:81], %[win]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_add_u32 s86, s84, 1\n\t"
-        "s_cbranch_scc1 9f\n\t"
         "s_lshr_b32 s85, %[kRP], 2\n\t"
-        "s_add_u32 s87, s86, s85\n\t"
+        "s_orn2_b64 s[80:81], 1, %[win]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s87, s84, s85\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 9f\n\t"
         "s_lshl_b64 s[80:81], %[win], s84\n\t"
@@ -376,17 +374,16 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_lshl_b64 s[80:81], %[win], 1\n\t"
-        "s_sub_u32 s88, 64, s85\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[90:91], %[win], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 37f\n\t"
-        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
-        "s_not_b64 s[80:81], s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_add_u32 s86, s85, 1\n\t"
+        "s_lshl_b64 s[92:93], %[win], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_add_u32 s86, s85, 2\n\t"
-        "s_add_u32 s86, s86, s84\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 37f\n\t"
@@ -414,17 +411,16 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_lshl_b64 s[80:81], %[win], 1\n\t"
-        "s_sub_u32 s88, 64, s85\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[90:91], %[win], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 47f\n\t"
-        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
-        "s_not_b64 s[80:81], s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_add_u32 s86, s85, 1\n\t"
+        "s_lshl_b64 s[92:93], %[win], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_add_u32 s86, s85, 2\n\t"
-        "s_add_u32 s86, s86, s84\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 47f\n\t"
@@ -452,17 +448,16 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_lshl_b64 s[80:81], %[win], 1\n\t"
-        "s_sub_u32 s88, 64, s85\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[90:91], %[win], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 67f\n\t"
-        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
-        "s_not_b64 s[80:81], s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_add_u32 s86, s85, 1\n\t"
+        "s_lshl_b64 s[92:93], %[win], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_add_u32 s86, s85, 2\n\t"
-        "s_add_u32 s86, s86, s84\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 67f\n\t"
@@ -490,17 +485,16 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_lshl_b64 s[80:81], %[win], 1\n\t"
-        "s_sub_u32 s88, 64, s85\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[90:91], %[win], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 77f\n\t"
-        "s_lshl_b64 s[92:93], s[80:81], s85\n\t"
-        "s_not_b64 s[80:81], s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_add_u32 s86, s85, 1\n\t"
+        "s_lshl_b64 s[92:93], %[win], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_add_u32 s86, s85, 2\n\t"
-        "s_add_u32 s86, s86, s84\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 77f\n\t"

@@ -170,6 +170,10 @@ class Loop:
                 v = (rd(a[1]) | rd(a[2])) & M64
                 wr(a[0], v)
                 scc = int(v != 0)
+            elif op == "s_orn2_b64":
+                v = (rd(a[1]) | ~rd(a[2])) & M64
+                wr(a[0], v)
+                scc = int(v != 0)
             elif op == "s_bitcmp1_b64":
                 scc = (rd(a[0]) >> (rd(a[1]) & 63)) & 1
             elif op.startswith("s_cmp_"):
