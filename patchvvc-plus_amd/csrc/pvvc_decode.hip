@@ -176,8 +176,13 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * value is never 0, so the count stays below 64 and s_orn2 sets SCC, which
  * s_flbit leaves for the s_addc that adds the code's terminating zero.  A
  * run's zero count is read straight from the window (its top bit is the
- * prefix's terminating zero).  Refills (11/13/51/55, 20, 33/43/63/73, 38)
- * are out of line.
+ * prefix's terminating zero).  Inside the loop k_P is held as k_P - 4 (set
+ * on entry, restored at the exit 9), so that the update itself is the mode
+ * test: a zero symbol's +3 carries exactly when k_P reaches 4 (run mode),
+ * and a run symbol's -1 borrows exactly when it falls to 3 (k = 0 mode);
+ * those updates follow the chunk test, so their chunk-end and rotation exits
+ * take them on the way out (85-88).  Refills (11/13/51/55, 20, 33/43/63/73,
+ * 38) are out of line.
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window, so every check precedes any state change.  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
@@ -190,8 +195,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
                                                uint32_t& kP, uint32_t& neg, uint32_t cur, int32_t& acc) {
     asm volatile(
         "s_mov_b32 m0, %[neg]\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc1 30f\n\t"
+        "s_sub_u32 %[kP], %[kP], 4\n\t"
+        "s_cbranch_scc0 30f\n\t"
         "10:\n\t"
         /* ---- k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
@@ -210,7 +215,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], 0\n\t"
+        "s_max_i32 %[kP], %[kP], -4\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 11f\n\t"
@@ -235,7 +240,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], 0\n\t"
+        "s_max_i32 %[kP], %[kP], -4\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 13f\n\t"
@@ -260,7 +265,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], 0\n\t"
+        "s_max_i32 %[kP], %[kP], -4\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 51f\n\t"
@@ -285,7 +290,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], 0\n\t"
+        "s_max_i32 %[kP], %[kP], -4\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 55f\n\t"
@@ -348,7 +353,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "20:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc1 88f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_nop 1\n\t"
@@ -361,24 +366,23 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "17:\n\t"
         "s_sub_i32 s90, s90, 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_add_u32 %[kP], %[kP], 3\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 20b\n\t"
         "21:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cbranch_scc1 86f\n\t"
+        "s_add_u32 %[kP], %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
         "30:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
         "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_sub_u32 s88, 63, s85\n\t"
+        "s_sub_u32 s88, 62, s85\n\t"
         "s_lshr_b64 s[90:91], %[win], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 37f\n\t"
-        "s_add_u32 s86, s85, 1\n\t"
+        "s_add_u32 s86, s85, 2\n\t"
         "s_lshl_b64 s[92:93], %[win], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
@@ -395,7 +399,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
@@ -403,19 +406,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 33f\n\t"
         "34:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc0 10b\n\t"
+        "s_cbranch_scc1 85f\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_cbranch_scc1 10b\n\t"
         "40:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
         "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_sub_u32 s88, 63, s85\n\t"
+        "s_sub_u32 s88, 62, s85\n\t"
         "s_lshr_b64 s[90:91], %[win], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 47f\n\t"
-        "s_add_u32 s86, s85, 1\n\t"
+        "s_add_u32 s86, s85, 2\n\t"
         "s_lshl_b64 s[92:93], %[win], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
@@ -432,7 +435,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
@@ -440,19 +442,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 43f\n\t"
         "44:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc0 10b\n\t"
+        "s_cbranch_scc1 85f\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_cbranch_scc1 10b\n\t"
         "60:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
         "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_sub_u32 s88, 63, s85\n\t"
+        "s_sub_u32 s88, 62, s85\n\t"
         "s_lshr_b64 s[90:91], %[win], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 67f\n\t"
-        "s_add_u32 s86, s85, 1\n\t"
+        "s_add_u32 s86, s85, 2\n\t"
         "s_lshl_b64 s[92:93], %[win], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
@@ -469,7 +471,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
@@ -477,19 +478,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 63f\n\t"
         "64:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc0 10b\n\t"
+        "s_cbranch_scc1 85f\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_cbranch_scc1 10b\n\t"
         "70:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
         "s_bitcmp1_b64 %[win], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_sub_u32 s88, 63, s85\n\t"
+        "s_sub_u32 s88, 62, s85\n\t"
         "s_lshr_b64 s[90:91], %[win], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 77f\n\t"
-        "s_add_u32 s86, s85, 1\n\t"
+        "s_add_u32 s86, s85, 2\n\t"
         "s_lshl_b64 s[92:93], %[win], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
@@ -506,7 +507,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
@@ -514,14 +514,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 73f\n\t"
         "74:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
-        "s_cbranch_scc0 10b\n\t"
+        "s_cbranch_scc1 85f\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_cbranch_scc1 10b\n\t"
         "s_branch 30b\n\t"
         "33:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc1 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_nop 1\n\t"
@@ -534,7 +534,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "43:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc1 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_nop 1\n\t"
@@ -553,7 +553,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "63:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc1 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_nop 1\n\t"
@@ -566,7 +566,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "73:\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc1 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_nop 1\n\t"
@@ -586,7 +586,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_not_b64 s[80:81], %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_mov_b32 s82, 0\n\t"
-        "s_mov_b32 s87, %[kP]\n\t"
+        "s_add_u32 s87, %[kP], 4\n\t"
         "31:\n\t"
         "s_cmp_gt_u32 s84, 6\n\t"
         "s_cbranch_scc1 9f\n\t"
@@ -628,7 +628,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_or_b32 s88, s88, s90\n\t"
         "s_add_u32 s88, s88, 1\n\t"
-        "s_sub_u32 %[kP], s87, 1\n\t"
+        "s_sub_u32 %[kP], s87, 5\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
@@ -637,7 +637,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "39:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
-        "s_cmp_gt_u32 %[kP], 3\n\t"
+        "s_cmp_ge_i32 %[kP], 0\n\t"
         "s_cbranch_scc0 10b\n\t"
         "s_branch 30b\n\t"
         "38:\n\t"
@@ -653,11 +653,23 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 39b\n\t"
+        "85:\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_branch 9f\n\t"
+        "86:\n\t"
+        "s_add_u32 %[kP], %[kP], 3\n\t"
+        "s_branch 9f\n\t"
+        "87:\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_branch 7f\n\t"
+        "88:\n\t"
+        "s_add_u32 %[kP], %[kP], 3\n\t"
         "7:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_branch 9f\n\t"
         "8:\n\t"
         "9:\n\t"
+        "s_add_u32 %[kP], %[kP], 4\n\t"
         "s_mov_b32 %[neg], m0\n\t"
 
         : [win] "+s"(win), [nbx] "+s"(nbx), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [neg] "+s"(neg),

@@ -179,7 +179,8 @@ class Loop:
             elif op.startswith("s_cmp_"):
                 x, y = rd(a[0]) & M32, rd(a[1]) & M32
                 scc = int({"s_cmp_gt_u32": x > y, "s_cmp_eq_u32": x == y, "s_cmp_lg_u32": x != y,
-                           "s_cmp_lt_u32": x < y, "s_cmp_ge_u32": x >= y}[op])
+                           "s_cmp_lt_u32": x < y, "s_cmp_ge_u32": x >= y,
+                           "s_cmp_ge_i32": sx(x) >= sx(y)}[op])
             elif op == "v_mov_b32":
                 v = rd(a[1]) & M32
                 s[a[0][2:-1]] = [v] * 64

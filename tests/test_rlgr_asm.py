@@ -34,7 +34,7 @@ def _cases(oracle, seed, streams_per_kind, states_per_stream):
                 yield kind, bits, pos, R.make_state(bits, pos, kP, kRP, rng, nb, wpos, lim, nrel)
 
 
-def _check(oracle, lines, seed=5, streams_per_kind=2, states_per_stream=30):
+def _check(oracle, lines, seed=5, streams_per_kind=3, states_per_stream=40):
     loop = R.Loop(lines)
     visited = set()
     fails = []
