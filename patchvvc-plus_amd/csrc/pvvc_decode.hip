@@ -181,7 +181,10 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * test: a zero symbol's +3 carries exactly when k_P reaches 4 (run mode),
  * and a run symbol's -1 borrows exactly when it falls to 3 (k = 0 mode);
  * those updates follow the chunk test, so their chunk-end and rotation exits
- * take them on the way out (85-88).  Refills (11/13/51/55, 20, 33/43/63/73,
+ * take them on the way out (85-88).  A non-zero k = 0 symbol's -1 is not
+ * clamped: k_P - 4 may fall below -4 in the k = 0 loop (by at most one chunk
+ * of symbols) and is clamped where it is read, in the zero block and at the
+ * exits.  Refills (11/13/51/55, 20, 33/43/63/73,
  * 38) are out of line.
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window, so every check precedes any state change.  The loop returns
@@ -215,7 +218,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], -4\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 11f\n\t"
@@ -240,7 +242,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], -4\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 13f\n\t"
@@ -265,7 +266,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], -4\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 51f\n\t"
@@ -290,7 +290,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_max_i32 %[kP], %[kP], -4\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 55f\n\t"
@@ -371,6 +370,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "21:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 86f\n\t"
+        "s_max_i32 %[kP], %[kP], -4\n\t"
         "s_add_u32 %[kP], %[kP], 3\n\t"
         "s_cbranch_scc0 10b\n\t"
         "30:\n\t"
@@ -657,18 +657,21 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_branch 9f\n\t"
         "86:\n\t"
+        "s_max_i32 %[kP], %[kP], -4\n\t"
         "s_add_u32 %[kP], %[kP], 3\n\t"
         "s_branch 9f\n\t"
         "87:\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_branch 7f\n\t"
         "88:\n\t"
+        "s_max_i32 %[kP], %[kP], -4\n\t"
         "s_add_u32 %[kP], %[kP], 3\n\t"
         "7:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_branch 9f\n\t"
         "8:\n\t"
         "9:\n\t"
+        "s_max_i32 %[kP], %[kP], -4\n\t"
         "s_add_u32 %[kP], %[kP], 4\n\t"
         "s_mov_b32 %[neg], m0\n\t"
 
