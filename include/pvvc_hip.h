@@ -182,6 +182,9 @@ typedef struct pvvc_tuning {
 
 int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_patch);
 int pvvc_decoder_destroy(pvvc_decoder* dec);
+/* Until a tuning is set (or the autotuner runs), a persistent segment of at most 4 slices per CU
+ * runs 4 wavefronts per slice with 32 KiB of LDS and 4 workgroups per CU; larger ones use the
+ * decoder's tuning (default 1 wavefront, 8 KiB, 16 workgroups per CU, queue chunk 1). */
 int pvvc_decoder_set_tuning(pvvc_decoder* dec, const pvvc_tuning* tuning);
 int pvvc_decoder_get_tuning(const pvvc_decoder* dec, pvvc_tuning* tuning);
 
