@@ -67,7 +67,12 @@ struct LaunchRes {
 
 /* Workgroups of a persistent launch of `count` items. */
 uint32_t persistent_grid(const pvvc_tuning& t, uint32_t count);
-/* Enqueues one segment: queue reset, persistent_kernel, finish pass. */
+/* The launch shape of a segment of `count` items: `t`, or, when auto_small
+ * and the segment has at most 4 items per CU, 4 wavefronts per item with
+ * 32 KiB of LDS and 4 workgroups per CU. */
+pvvc_tuning segment_tuning(const pvvc_tuning& t, bool auto_small, uint32_t count);
+/* Enqueues one segment: persistent_kernel (its last workgroup re-arms the
+ * queue) and the finish pass. */
 void launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, const pvvc_tuning& t,
                     const LaunchRes& r, uint32_t launch_id, hipStream_t s);
 /* Records a HIP failure in the library's sticky error; true on success. */
