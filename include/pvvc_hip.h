@@ -67,9 +67,29 @@ typedef struct pvvc_state pvvc_state;
  * points per patch.  Returns NULL on failure (pvvc_last_error() says why). */
 pvvc_state* pvvc_state_alloc(int n_patches, int max_points);
 void pvvc_state_free(pvvc_state* state);
-/* Status of patch `patch` from the last decode that touched it (D2H read,
- * synchronises the library stream). */
+/* Status of patch `patch` from the last decode that touched it.  Synchronises
+ * the state's stream and, like pvvc_state_synchronize, moves the state's
+ * pending launch error into the calling thread's pvvc_last_error(). */
 int pvvc_state_patch_status(const pvvc_state* state, int patch);
+
+/* Per-state settings of the drop-in call.  Each state object carries its own
+ * stream, launch shape and white-small flag, so host threads that each drive
+ * their own state never share mutable library state; a setting a state has
+ * not made follows the process-wide default (pvvc_set_stream,
+ * pvvc_set_dropin_tuning, pvvc_set_white_small_patches).  The reference
+ * creates and destroys a stream inside every call
+ * (src/cuda/render/CUDA2OpenGL.cu:69-73), so its callers never share one. */
+int pvvc_state_set_stream(pvvc_state* state, void* stream);
+/* waves 0 returns the state to the process default. */
+int pvvc_state_set_dropin_tuning(pvvc_state* state, int waves, int lds_bytes);
+/* enable -1 returns the state to the process default. */
+int pvvc_state_set_white_small_patches(pvvc_state* state, int enable);
+/* Waits for the state's stream, then returns (and records in the calling
+ * thread's pvvc_last_error) the status of the first slice that failed in the
+ * drop-in calls since the previous read: PVVC_ERR_INVALID_ARG for an index[s]
+ * outside [0, n_patches), else that slice's per-patch code; PVVC_OK if none.
+ * The reference reports nothing (its kernel fails silently). */
+int pvvc_state_synchronize(pvvc_state* state);
 
 /* Drop-in for launch_cudaProcess (include/cuda/render.cuh:41-57,
  * src/cuda/render/CUDA2OpenGL.cu:68-74).  Same argument meaning: every
@@ -77,26 +97,37 @@ int pvvc_state_patch_status(const pvvc_state* state, int patch);
  * already zstd-unwrapped bytes of slice s; mv[s] is 16 row-major floats;
  * inner_offset[s] is the output offset of slice s; the state array is
  * indexed by index[s].  grid and block are accepted and ignored (the
- * library picks the launch shape).  Enqueued asynchronously on the stream
- * set by pvvc_set_stream (default: the null stream); the caller
- * synchronises, as the reference callers do (test/test_decode_only.cc:147).
- * The reference returns nothing and reports nothing; failures here are
- * recorded per patch (pvvc_state_patch_status) and in pvvc_last_error(). */
+ * library picks the launch shape).  Enqueued asynchronously on the state's
+ * stream (pvvc_state_set_stream; else the process default of
+ * pvvc_set_stream; else the null stream); the caller synchronises, as the
+ * reference callers do (test/test_decode_only.cc:147).  Calls on different
+ * state objects are independent and may come from different threads.
+ * The reference returns nothing and reports nothing; here a slice's failure
+ * is recorded per patch (pvvc_state_patch_status), and the first failure of
+ * the state's calls (including an index[s] out of range, whose slice is
+ * skipped) is returned by pvvc_state_synchronize and folded into the calling
+ * thread's pvvc_last_error() by pvvc_state_synchronize,
+ * pvvc_state_patch_status and, for states on the default stream,
+ * pvvc_synchronize. */
 void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, int* inner_offset, int* index,
                         uint8_t* type, float** mv, uint32_t* size, uint8_t* qp, uint8_t** geometry,
                         uint32_t* geometry_size, uint8_t** color, uint32_t* color_size, pvvc_state* invertRAHTOctree_gpu,
                         int patch_size);
 
-/* Launch shape of launch_cudaProcess: wavefronts per slice (1, 2, 4) and the
- * LDS workspace per slice in bytes (arrays that do not fit use the state
- * slot's global workspace).  Default: 4 wavefronts, 32 KiB. */
+/* Process-wide default launch shape of launch_cudaProcess, for states that
+ * set none: wavefronts per slice (1, 2, 4) and the LDS workspace per slice in
+ * bytes (arrays that do not fit use the state slot's global workspace).
+ * Default: 4 wavefronts, 32 KiB. */
 int pvvc_set_dropin_tuning(int waves, int lds_bytes);
-/* Stream used by launch_cudaProcess and the native calls (a hipStream_t). */
+/* Process-wide default stream (a hipStream_t) of the states and decoders that
+ * set none, of the encoder, the metric and the helper copies. */
 int pvvc_set_stream(void* stream);
-/* Last error recorded by any call in this process (sticky until read). */
+/* Last error recorded by a call made on the calling thread (cleared when
+ * read).  Each host thread has its own. */
 int pvvc_last_error(void);
 /* Reference-only rendering artefact: patches with fewer than 10 points
- * forced to white (src/cuda/render/CUDA2OpenGL.cu:54-58).  Off by default. */
+ * forced to white (src/cuda/render/CUDA2OpenGL.cu:54-58).  Process-wide
+ * default, off. */
 int pvvc_set_white_small_patches(int enable);
 
 /* ------------------------------------------------------------------------ */
@@ -153,7 +184,7 @@ void pvvc_dropin_release(pvvc_dropin_frame* dev);
 size_t pvvc_dropin_stage_bytes(const pvvc_frame* frame);
 /* Stages into caller buffers, allocating nothing: packs the image into
  * `host` (pinned memory from pvvc_host_alloc makes the copy asynchronous)
- * and copies it to `dev` on the library stream.  `host` may be reused once
+ * and copies it to `dev` on the default stream.  `host` may be reused once
  * the stream has passed the copy.  PVVC_ERR_CAPACITY if either buffer is
  * smaller than pvvc_dropin_stage_bytes(frame). */
 int pvvc_dropin_stage_to(const pvvc_frame* frame, void* dev, size_t dev_bytes, void* host, size_t host_bytes,
@@ -187,6 +218,11 @@ int pvvc_decoder_destroy(pvvc_decoder* dec);
  * decoder's tuning (default 1 wavefront, 8 KiB, 16 workgroups per CU, queue chunk 1). */
 int pvvc_decoder_set_tuning(pvvc_decoder* dec, const pvvc_tuning* tuning);
 int pvvc_decoder_get_tuning(const pvvc_decoder* dec, pvvc_tuning* tuning);
+/* The decoder's own stream (a hipStream_t) for every call on it and its
+ * batches; until set, the process default (pvvc_set_stream). */
+int pvvc_decoder_set_stream(pvvc_decoder* dec, void* stream);
+/* The decoder's own white-small setting (-1: the process default). */
+int pvvc_decoder_set_white_small_patches(pvvc_decoder* dec, int enable);
 
 /* Packs frames [0, n_frames) (in decode order) into one device-resident
  * batch: a single host->device copy.  Work items are all slices of all
@@ -213,7 +249,7 @@ enum pvvc_mode {
     PVVC_MODE_GRAPH = 2        /* per-frame launches replayed from one captured hipGraph per frame */
 };
 
-/* Decodes the batch asynchronously on the library stream.  Output for
+/* Decodes the batch asynchronously on the decoder's stream.  Output for
  * frame f starts at point pvvc_batch_frame_offset(f).  Native output:
  * xyz (float x3) and rgb (uint8 x3) per point; Points output: `points`. */
 int pvvc_decode_batch(pvvc_decoder* dec, pvvc_batch* batch, int mode, int output, float* xyz, uint8_t* rgb,
@@ -241,7 +277,7 @@ int pvvc_decoder_reset(pvvc_decoder* dec);
  * chunk by coordinate descent, timing `reps` decodes of `batch` per
  * candidate (HIP events; median).  Sets the decoder's tuning to the best
  * and returns it with its time; the outputs hold that configuration's
- * decode.  Synchronises the library stream. */
+ * decode.  Synchronises the decoder's stream. */
 int pvvc_decoder_autotune(pvvc_decoder* dec, pvvc_batch* batch, int mode, int output, float* xyz, uint8_t* rgb,
                           Points* points, int reps, pvvc_tuning* best, double* best_ms);
 
@@ -381,12 +417,19 @@ void* pvvc_dev_alloc(size_t bytes);
 void pvvc_dev_free(void* ptr);
 int pvvc_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int pvvc_memcpy_d2h(void* dst, const void* src, size_t bytes);
-/* Device-to-device copy on the library stream (asynchronous). */
+/* Device-to-device copy on the default stream (asynchronous). */
 int pvvc_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int pvvc_memset(void* dst, int value, size_t bytes);
+/* Waits for the default stream, then folds the pending launch errors of the
+ * states on that stream into the calling thread's pvvc_last_error(). */
 int pvvc_synchronize(void);
+/* A new non-blocking HIP stream at the default priority (for callers without a
+ * HIP binding of their own); destroy it after its last use. */
+int pvvc_stream_create(void** stream);
+int pvvc_stream_destroy(void* stream);
+int pvvc_stream_synchronize(void* stream);
 /* Kernel time of the last decode call made on the calling thread, from HIP
- * events recorded on the library stream around its launches (milliseconds;
+ * events recorded on the launch stream around its launches (milliseconds;
  * includes the queue-reset and finish launches of a persistent decode). */
 double pvvc_last_kernel_ms(void);
 /* Per-launch times (ms) of the last decode call on the calling thread;

@@ -1132,13 +1132,14 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
         if (!a.geo || a.geo_size < 25) st = PVVC_ERR_BAD_SLICE;
         else if (a.n > slot->n_cap || a.geo_size > slot->geo_cap) st = PVVC_ERR_CAPACITY;
     } else if (!wait) {
+        /* a predictive slice decodes on its patch's intra tree: the reference
+         * builds the tree only for intra slices (invert_raht_octree.cc:71-78)
+         * and ignores any geometry bytes a predictive slice carries */
         const uint32_t sn = slot->n;
         if (sn == 0) st = PVVC_ERR_EMPTY_REFERENCE;
         else if (sn != a.n) st = PVVC_ERR_UNMATCHED_SIZE;
-        else if (!a.geo || a.geo_size == 0) {
-            a.geo = slot->geo;
-            a.geo_size = slot->geo_size;
-        }
+        a.geo = slot->geo;
+        a.geo_size = slot->geo_size;
     } else if (a.n > slot->n_cap || !a.geo || a.geo_size < 25) {
         /* deferred: the intra slice decodes in this launch; validity against
          * it is settled by the finish pass */
@@ -1355,8 +1356,9 @@ __global__ __launch_bounds__(64 * WAVES) void dropin_kernel(Points* out, const i
     const int s = blockIdx.x;
     if (s >= patch_size) return;
     const int pidx = index[s];
+    /* err_flag keeps the first failure's status until the host reads it */
     if (pidx < 0 || pidx >= n_slots) {
-        if (threadIdx.x == 0) atomicOr(err_flag, 1);
+        if (threadIdx.x == 0) atomicCAS(err_flag, 0, static_cast<int>(PVVC_ERR_INVALID_ARG));
         return;
     }
     SlotDev* slot = slots + pidx;
@@ -1378,7 +1380,7 @@ __global__ __launch_bounds__(64 * WAVES) void dropin_kernel(Points* out, const i
     Sink sink{nullptr, nullptr, out, PVVC_OUT_POINTS, white_small};
     Arena ar{g_smem, lds_bytes, 0, slot->ws, slot->ws_cap, 0};
     int st = decode_slice<WAVES>(a, slot, ar, sink, meta, 0u, false, static_cast<uint32_t>(s));
-    if (st != PVVC_OK && threadIdx.x == 0) atomicOr(err_flag, 2);
+    if (st != PVVC_OK && threadIdx.x == 0) atomicCAS(err_flag, 0, st);
 }
 
 __device__ __forceinline__ SliceArgs item_args(const ItemDev* __restrict__ it) {
@@ -1869,14 +1871,20 @@ using namespace pvvc;
 
 namespace {
 
-std::atomic<int> g_last_error{0};
-hipStream_t g_stream = nullptr;
-int g_white_small = 0;
+/* The last error is per host thread: two threads driving their own states
+ * or decoders never see each other's failures. */
+thread_local int t_last_error = 0;
+/* Process-wide defaults, read by every object that has not set its own
+ * (pvvc_state_set_*, pvvc_decoder_set_*). */
+std::atomic<void*> g_default_stream{nullptr};
+std::atomic<int> g_white_small{0};
 std::mutex g_mu;
 
 void set_error(int e) {
-    if (e) g_last_error.store(e);
+    if (e) t_last_error = e;
 }
+
+hipStream_t default_stream() { return static_cast<hipStream_t>(g_default_stream.load()); }
 
 bool hip_ok(hipError_t e) {
     if (e != hipSuccess) {
@@ -1929,17 +1937,16 @@ struct Timing {
 thread_local Timing g_timing;   /* whole decode call: every launch of it, in order */
 thread_local Timing g_ktiming;  /* the decode kernel launches alone (frame / persistent / drop-in) */
 
-struct TuningState {
-    int waves = 1;
-    int lds_bytes = 24 * 1024;
-    int wgs_per_cu = 8;
-    int queue_chunk = 1;
-};
-/* 4 waves per slice: the second builds the octree while the first decodes
- * RLGR, and all four share the top-down passes and the leaf output; on a
- * 10-bit 816k-point frame of 820 slices this measured 0.521 ms against
- * 0.533 ms for 2 waves and 0.618 ms for 1 (16-32 KiB of LDS alike) */
-TuningState g_dropin_tuning{4, 32 * 1024, 8, 1};
+/* Drop-in launch shape (waves per slice, LDS bytes per slice), packed in one
+ * word so that readers never see half an update.  Default 4 waves: the second
+ * builds the octree while the first decodes RLGR, and all four share the
+ * top-down passes and the leaf output; on a 10-bit 816k-point frame of 820
+ * slices this measured 0.521 ms against 0.533 ms for 2 waves and 0.618 ms
+ * for 1 (16-32 KiB of LDS alike) */
+constexpr uint64_t pack_dropin(int waves, int lds) {
+    return (static_cast<uint64_t>(static_cast<uint32_t>(waves)) << 32) | static_cast<uint32_t>(lds);
+}
+std::atomic<uint64_t> g_dropin_tuning{pack_dropin(4, 32 * 1024)};
 
 int clamp_lds(int want) {
     int mx = devinfo().lds_max > 0 ? devinfo().lds_max : 65536;
@@ -1962,9 +1969,35 @@ struct StateInfo {
     uint32_t geo_cap = 0;
     uint64_t ws_cap = 0;
     uint64_t slot_stride = 0;
-    int* err = nullptr;        /* device error word */
+    int* err = nullptr;        /* device word: status of the first failed slice since the last read */
+    /* per-state settings; unset ones follow the process-wide defaults */
+    bool has_stream = false;
+    hipStream_t stream = nullptr;
+    uint64_t dropin = 0;       /* pack_dropin(waves, lds), 0 = default */
+    int white_small = -1;      /* -1 = default */
 };
 std::map<const void*, StateInfo> g_states;
+
+hipStream_t state_stream(const StateInfo& si) { return si.has_stream ? si.stream : default_stream(); }
+
+/* Reads and clears a state's device error word on stream s (synchronising s). */
+int take_state_error(const StateInfo& si, hipStream_t s) {
+    int v = 0;
+    if (!hip_ok(hipMemcpyAsync(&v, si.err, sizeof(int), hipMemcpyDeviceToHost, s)) ||
+        !hip_ok(hipStreamSynchronize(s)))
+        return PVVC_ERR_HIP;
+    if (v && !(hip_ok(hipMemsetAsync(si.err, 0, sizeof(int), s)) && hip_ok(hipStreamSynchronize(s))))
+        return PVVC_ERR_HIP;
+    return v;
+}
+
+bool find_state(const void* state, StateInfo& out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_states.find(state);
+    if (it == g_states.end()) return false;
+    out = it->second;
+    return true;
+}
 
 uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
@@ -1983,12 +2016,9 @@ bool state_create(int n_patches, int max_points, StateInfo& si) {
     si.ws_cap = ws_cap_for(max_points);
     si.slot_stride = align256(si.geo_cap) + align256(12ull * max_points) + si.ws_cap;
     if (!hip_ok(hipMalloc(&si.slots, sizeof(SlotDev) * std::max(1, n_patches)))) return false;
-    if (!hip_ok(hipMalloc(&si.arena, si.slot_stride * std::max(1, n_patches)))) {
-        hipFree(si.slots);
-        return false;
-    }
-    if (!hip_ok(hipMalloc(&si.err, sizeof(int)))) return false;
-    hipMemset(si.err, 0, sizeof(int));
+    /* on failure the caller's state_destroy frees what was allocated */
+    if (!hip_ok(hipMalloc(&si.arena, si.slot_stride * std::max(1, n_patches)))) return false;
+    if (!hip_ok(hipMalloc(&si.err, sizeof(int))) || !hip_ok(hipMemset(si.err, 0, sizeof(int)))) return false;
     std::vector<SlotDev> h(std::max(1, n_patches));
     for (int i = 0; i < n_patches; ++i) {
         SlotDev& s = h[i];
@@ -2209,7 +2239,15 @@ struct pvvc_decoder {
     uint64_t stage_bytes = 0;
     std::vector<uint32_t> intra_geo; /* geometry bytes of each patch's latest intra slice
                                         seen by pvvc_batch_create (state-read accounting) */
+    bool has_stream = false;     /* pvvc_decoder_set_stream; else the process default */
+    hipStream_t stream = nullptr;
+    int white_small = -1;        /* pvvc_decoder_set_white_small_patches; -1 = process default */
 };
+
+namespace {
+hipStream_t dec_stream(const pvvc_decoder* d) { return d->has_stream ? d->stream : default_stream(); }
+int dec_white_small(const pvvc_decoder* d) { return d->white_small >= 0 ? d->white_small : g_white_small.load(); }
+}  // namespace
 
 struct pvvc_batch {
     BatchPlan plan;
@@ -2227,9 +2265,15 @@ struct pvvc_batch {
     uint32_t* order = nullptr;
     uint32_t* defer = nullptr;
     int* status = nullptr;
+    hipStream_t last_stream = nullptr; /* stream of the last decode of this batch */
+    /* captured per-frame launches, valid for exactly this key: output mode and
+     * buffers, waves per slice, LDS bytes and the white-small setting */
     hipGraphExec_t graph = nullptr;
     int graph_output = -1;
     void* graph_ptrs[3] = {nullptr, nullptr, nullptr};
+    int graph_waves = 0;
+    uint32_t graph_lds = 0;
+    int graph_white = -1;
 };
 
 namespace {
@@ -2278,7 +2322,7 @@ void rt_timing_reset() {
     g_ktiming.reset();
 }
 void rt_set_error(int e) { set_error(e); }
-hipStream_t rt_stream() { return g_stream; }
+hipStream_t rt_stream() { return default_stream(); }
 
 void rt_init_kernels() {
     static std::once_flag once;
@@ -2327,11 +2371,16 @@ pvvc_tuning segment_tuning(const pvvc_tuning& t, bool auto_small, uint32_t count
     return e;
 }
 
-void launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, const pvvc_tuning& t, const LaunchRes& r,
-                    uint32_t launch_id, hipStream_t s) {
-    if (!sg.count) return;
+int launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, const pvvc_tuning& t, const LaunchRes& r,
+                   uint32_t launch_id, hipStream_t s) {
+    if (!sg.count) return PVVC_OK;
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(t.lds_bytes));
     const uint32_t grid = persistent_grid(t, sg.count);
+    /* workgroup g uses slab bytes [g * stride, g * stride + stride) */
+    if (sg.ws_need > r.slab_bytes || static_cast<uint64_t>(grid) * r.slab_bytes > r.slab_total) {
+        set_error(PVVC_ERR_CAPACITY);
+        return PVVC_ERR_CAPACITY;
+    }
     const uint32_t chunk = static_cast<uint32_t>(std::max(1, t.queue_chunk));
     /* default: the first wave of the queue, one item per workgroup, at the
      * top priority.  Those items start together; the scalar-issue arbiter
@@ -2345,13 +2394,18 @@ void launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, cons
     if (t.waves_per_patch == 2) launch_persistent_w<2>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
     else if (t.waves_per_patch == 4) launch_persistent_w<4>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
     else launch_persistent_w<1>(b, sg, sink, lds, grid, chunk, prio, r, launch_id, s);
+    return PVVC_OK;
 }
 
 }  // namespace pvvc
 
 extern "C" {
 
-int pvvc_last_error(void) { return g_last_error.exchange(0); }
+int pvvc_last_error(void) {
+    const int e = t_last_error;
+    t_last_error = 0;
+    return e;
+}
 
 #ifdef PVVC_PHASE_PROFILE
 /* Profiling build only: installs the stamp buffer (device, 8 u64 per item). */
@@ -2361,12 +2415,12 @@ int pvvc_phase_profile_set(void* dev_buf) {
 #endif
 
 int pvvc_set_stream(void* stream) {
-    g_stream = static_cast<hipStream_t>(stream);
+    g_default_stream.store(stream);
     return PVVC_OK;
 }
 
 int pvvc_set_white_small_patches(int enable) {
-    g_white_small = enable ? 1 : 0;
+    g_white_small.store(enable ? 1 : 0);
     return PVVC_OK;
 }
 
@@ -2395,12 +2449,45 @@ int pvvc_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)) ? PVVC_OK : PVVC_ERR_HIP;
 }
 int pvvc_memcpy_d2d(void* dst, const void* src, size_t bytes) {
-    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream)) ? PVVC_OK : PVVC_ERR_HIP;
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, default_stream())) ? PVVC_OK
+                                                                                              : PVVC_ERR_HIP;
 }
 int pvvc_memset(void* dst, int value, size_t bytes) {
     return hip_ok(hipMemset(dst, value, bytes)) ? PVVC_OK : PVVC_ERR_HIP;
 }
-int pvvc_synchronize(void) { return hip_ok(hipStreamSynchronize(g_stream)) ? PVVC_OK : PVVC_ERR_HIP; }
+/* Synchronises the default stream, then folds the device error words of the
+ * states that run on it into the calling thread's last error. */
+int pvvc_synchronize(void) {
+    const hipStream_t s = default_stream();
+    if (!hip_ok(hipStreamSynchronize(s))) return PVVC_ERR_HIP;
+    std::vector<StateInfo> on_default;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (auto& kv : g_states)
+            if (!kv.second.has_stream) on_default.push_back(kv.second);
+    }
+    int first = PVVC_OK;
+    for (const StateInfo& si : on_default) {
+        const int e = take_state_error(si, s);
+        if (e && first == PVVC_OK) first = e;
+    }
+    set_error(first);
+    return PVVC_OK;
+}
+
+int pvvc_stream_create(void** stream) {
+    if (!stream) return PVVC_ERR_INVALID_ARG;
+    hipStream_t s = nullptr;
+    if (!hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking))) return PVVC_ERR_HIP;
+    *stream = s;
+    return PVVC_OK;
+}
+int pvvc_stream_destroy(void* stream) {
+    return hip_ok(hipStreamDestroy(static_cast<hipStream_t>(stream))) ? PVVC_OK : PVVC_ERR_HIP;
+}
+int pvvc_stream_synchronize(void* stream) {
+    return hip_ok(hipStreamSynchronize(static_cast<hipStream_t>(stream))) ? PVVC_OK : PVVC_ERR_HIP;
+}
 
 /* Milliseconds of the main kernel launches of the last decode call. */
 double pvvc_last_kernel_ms(void) {
@@ -2456,23 +2543,52 @@ void pvvc_state_free(pvvc_state* state) {
 
 int pvvc_state_patch_status(const pvvc_state* state, int patch) {
     StateInfo si;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        auto it = g_states.find(state);
-        if (it == g_states.end()) return PVVC_ERR_INVALID_ARG;
-        si = it->second;
-    }
+    if (!find_state(state, si)) return PVVC_ERR_INVALID_ARG;
     if (patch < 0 || patch >= si.n) return PVVC_ERR_INVALID_ARG;
-    if (!hip_ok(hipStreamSynchronize(g_stream))) return PVVC_ERR_HIP;
-    SlotDev s;
-    if (!hip_ok(hipMemcpy(&s, si.slots + patch, sizeof(s), hipMemcpyDeviceToHost))) return PVVC_ERR_HIP;
-    return s.status;
+    const hipStream_t s = state_stream(si);
+    set_error(take_state_error(si, s)); /* synchronises the state's stream */
+    SlotDev sd;
+    if (!hip_ok(hipMemcpy(&sd, si.slots + patch, sizeof(sd), hipMemcpyDeviceToHost))) return PVVC_ERR_HIP;
+    return sd.status;
+}
+
+int pvvc_state_set_stream(pvvc_state* state, void* stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_states.find(state);
+    if (it == g_states.end()) return PVVC_ERR_INVALID_ARG;
+    it->second.has_stream = true;
+    it->second.stream = static_cast<hipStream_t>(stream);
+    return PVVC_OK;
+}
+
+int pvvc_state_set_dropin_tuning(pvvc_state* state, int waves, int lds_bytes) {
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4) return PVVC_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_states.find(state);
+    if (it == g_states.end()) return PVVC_ERR_INVALID_ARG;
+    it->second.dropin = waves ? pack_dropin(waves, lds_bytes) : 0;
+    return PVVC_OK;
+}
+
+int pvvc_state_set_white_small_patches(pvvc_state* state, int enable) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_states.find(state);
+    if (it == g_states.end()) return PVVC_ERR_INVALID_ARG;
+    it->second.white_small = enable < 0 ? -1 : (enable ? 1 : 0);
+    return PVVC_OK;
+}
+
+int pvvc_state_synchronize(pvvc_state* state) {
+    StateInfo si;
+    if (!find_state(state, si)) return PVVC_ERR_INVALID_ARG;
+    const int e = take_state_error(si, state_stream(si));
+    set_error(e);
+    return e;
 }
 
 int pvvc_set_dropin_tuning(int waves, int lds_bytes) {
     if (waves != 1 && waves != 2 && waves != 4) return PVVC_ERR_INVALID_ARG;
-    g_dropin_tuning.waves = waves;
-    g_dropin_tuning.lds_bytes = lds_bytes;
+    g_dropin_tuning.store(pack_dropin(waves, lds_bytes));
     return PVVC_OK;
 }
 
@@ -2485,42 +2601,41 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
     (void)timestamp;
     if (patch_size <= 0) return;
     StateInfo si;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        auto it = g_states.find(state);
-        if (it == g_states.end()) {
-            set_error(PVVC_ERR_INVALID_ARG);
-            return;
-        }
-        si = it->second;
+    if (!find_state(state, si)) {
+        set_error(PVVC_ERR_INVALID_ARG);
+        return;
     }
-    const int lds = clamp_lds(g_dropin_tuning.lds_bytes);
+    const uint64_t shape = si.dropin ? si.dropin : g_dropin_tuning.load();
+    const int waves = static_cast<int>(shape >> 32);
+    const int lds = clamp_lds(static_cast<int>(static_cast<uint32_t>(shape)));
+    const int white = si.white_small >= 0 ? si.white_small : g_white_small.load();
+    const hipStream_t s = state_stream(si);
     rt_init_kernels();
     g_timing.reset();
     g_ktiming.reset();
     auto ev = g_timing.next();
     auto kev = g_ktiming.next();
-    hipEventRecord(ev.first, g_stream);
-    hipEventRecord(kev.first, g_stream);
-    switch (g_dropin_tuning.waves) {
+    hipEventRecord(ev.first, s);
+    hipEventRecord(kev.first, s);
+    switch (waves) {
         case 2:
-            hipLaunchKernelGGL(dropin_kernel<2>, dim3(patch_size), dim3(128), lds, g_stream, cudaData, inner_offset,
-                               index, type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
-                               patch_size, static_cast<uint32_t>(lds), g_white_small, si.err);
+            hipLaunchKernelGGL(dropin_kernel<2>, dim3(patch_size), dim3(128), lds, s, cudaData, inner_offset, index,
+                               type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
+                               patch_size, static_cast<uint32_t>(lds), white, si.err);
             break;
         case 4:
-            hipLaunchKernelGGL(dropin_kernel<4>, dim3(patch_size), dim3(256), lds, g_stream, cudaData, inner_offset,
-                               index, type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
-                               patch_size, static_cast<uint32_t>(lds), g_white_small, si.err);
+            hipLaunchKernelGGL(dropin_kernel<4>, dim3(patch_size), dim3(256), lds, s, cudaData, inner_offset, index,
+                               type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
+                               patch_size, static_cast<uint32_t>(lds), white, si.err);
             break;
         default:
-            hipLaunchKernelGGL(dropin_kernel<1>, dim3(patch_size), dim3(64), lds, g_stream, cudaData, inner_offset,
-                               index, type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
-                               patch_size, static_cast<uint32_t>(lds), g_white_small, si.err);
+            hipLaunchKernelGGL(dropin_kernel<1>, dim3(patch_size), dim3(64), lds, s, cudaData, inner_offset, index,
+                               type, mv, size, qp, geometry, geometry_size, color, color_size, si.slots, si.n,
+                               patch_size, static_cast<uint32_t>(lds), white, si.err);
     }
     hip_ok(hipGetLastError());
-    hipEventRecord(kev.second, g_stream);
-    hipEventRecord(ev.second, g_stream);
+    hipEventRecord(kev.second, s);
+    hipEventRecord(ev.second, s);
 }
 
 /* ---- drop-in staging (test_decode_only.cc:97-123 in one copy) ---- */
@@ -2618,7 +2733,7 @@ int pvvc_dropin_stage_to(const pvvc_frame* f, void* dev, size_t dev_bytes, void*
     const StageLayout L(f);
     if (L.total > dev_bytes || L.total > host_bytes) return PVVC_ERR_CAPACITY;
     stage_fill(f, L, static_cast<uint8_t*>(dev), static_cast<uint8_t*>(host), d);
-    if (!hip_ok(hipMemcpyAsync(dev, host, L.total, hipMemcpyHostToDevice, g_stream))) {
+    if (!hip_ok(hipMemcpyAsync(dev, host, L.total, hipMemcpyHostToDevice, default_stream()))) {
         std::memset(d, 0, sizeof(*d));
         return PVVC_ERR_HIP;
     }
@@ -2662,7 +2777,7 @@ int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_
 
 int pvvc_decoder_destroy(pvvc_decoder* d) {
     if (!d) return PVVC_ERR_INVALID_ARG;
-    hipStreamSynchronize(g_stream);
+    hipStreamSynchronize(dec_stream(d));
     state_destroy(d->st);
     if (d->slab) hipFree(d->slab);
     if (d->queue) hipFree(d->queue);
@@ -2693,9 +2808,22 @@ int pvvc_decoder_get_tuning(const pvvc_decoder* d, pvvc_tuning* t) {
     return PVVC_OK;
 }
 
+int pvvc_decoder_set_stream(pvvc_decoder* d, void* stream) {
+    if (!d) return PVVC_ERR_INVALID_ARG;
+    d->has_stream = true;
+    d->stream = static_cast<hipStream_t>(stream);
+    return PVVC_OK;
+}
+
+int pvvc_decoder_set_white_small_patches(pvvc_decoder* d, int enable) {
+    if (!d) return PVVC_ERR_INVALID_ARG;
+    d->white_small = enable < 0 ? -1 : (enable ? 1 : 0);
+    return PVVC_OK;
+}
+
 int pvvc_decoder_reset(pvvc_decoder* d) {
     if (!d) return PVVC_ERR_INVALID_ARG;
-    if (!hip_ok(hipStreamSynchronize(g_stream))) return PVVC_ERR_HIP;
+    if (!hip_ok(hipStreamSynchronize(dec_stream(d)))) return PVVC_ERR_HIP;
     return state_reset(d->st) ? PVVC_OK : PVVC_ERR_HIP;
 }
 
@@ -2722,6 +2850,7 @@ int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_fr
         return PVVC_ERR_HIP;
     }
     b->dec = d;
+    b->last_stream = dec_stream(d);
     b->n_frames = n_frames;
     b->n_items = pl.n_items;
     b->total_points = pl.total_points;
@@ -2742,7 +2871,7 @@ int pvvc_batch_create(pvvc_decoder* d, const pvvc_frame* const* frames, int n_fr
 
 void pvvc_batch_free(pvvc_batch* b) {
     if (!b) return;
-    hipStreamSynchronize(g_stream);
+    hipStreamSynchronize(b->last_stream);
     if (b->graph) hipGraphExecDestroy(b->graph);
     if (b->dev) hipFree(b->dev);
     delete b;
@@ -2761,7 +2890,7 @@ int pvvc_batch_items(const pvvc_batch* b) { return b ? static_cast<int>(b->n_ite
 
 int pvvc_batch_status(pvvc_batch* b, int* status, int n) {
     if (!b || !status) return PVVC_ERR_INVALID_ARG;
-    if (!hip_ok(hipStreamSynchronize(g_stream))) return PVVC_ERR_HIP;
+    if (!hip_ok(hipStreamSynchronize(b->last_stream))) return PVVC_ERR_HIP;
     int k = std::min<int>(n, static_cast<int>(b->n_items));
     return hip_ok(hipMemcpy(status, b->status, 4ull * k, hipMemcpyDeviceToHost)) ? PVVC_OK : PVVC_ERR_HIP;
 }
@@ -2772,7 +2901,10 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
     if (output == PVVC_OUT_NATIVE && (!xyz || !rgb)) return PVVC_ERR_INVALID_ARG;
     if (output == PVVC_OUT_POINTS && !points) return PVVC_ERR_INVALID_ARG;
     rt_init_kernels();
-    Sink sink{xyz, rgb, points, output, g_white_small};
+    const hipStream_t s = dec_stream(d);
+    b->last_stream = s;
+    const int white = dec_white_small(d);
+    Sink sink{xyz, rgb, points, output, white};
     const int W = d->tuning.waves_per_patch;
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(d->tuning.lds_bytes));
     g_timing.reset();
@@ -2780,70 +2912,81 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
     if (mode == PVVC_MODE_PER_FRAME || mode == PVVC_MODE_GRAPH) {
         if (mode == PVVC_MODE_GRAPH) {
             void* ptrs[3] = {xyz, rgb, points};
-            bool same = b->graph && b->graph_output == output && std::equal(ptrs, ptrs + 3, b->graph_ptrs);
+            const bool same = b->graph && b->graph_output == output && std::equal(ptrs, ptrs + 3, b->graph_ptrs) &&
+                              b->graph_waves == W && b->graph_lds == lds && b->graph_white == white;
             if (!same) {
                 if (b->graph) hipGraphExecDestroy(b->graph);
                 b->graph = nullptr;
                 hipStream_t cs;
                 if (!hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking))) return PVVC_ERR_HIP;
-                hipGraph_t g;
-                if (!hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal))) return PVVC_ERR_HIP;
-                for (int f = 0; f < b->n_frames; ++f) {
-                    if (W == 2) launch_frame<2>(b, f, sink, lds, cs);
-                    else if (W == 4) launch_frame<4>(b, f, sink, lds, cs);
-                    else launch_frame<1>(b, f, sink, lds, cs);
+                hipGraph_t g = nullptr;
+                bool good = hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+                if (good) {
+                    for (int f = 0; f < b->n_frames; ++f) {
+                        if (W == 2) launch_frame<2>(b, f, sink, lds, cs);
+                        else if (W == 4) launch_frame<4>(b, f, sink, lds, cs);
+                        else launch_frame<1>(b, f, sink, lds, cs);
+                    }
+                    good = hip_ok(hipStreamEndCapture(cs, &g));
                 }
-                if (!hip_ok(hipStreamEndCapture(cs, &g))) return PVVC_ERR_HIP;
-                if (!hip_ok(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0))) return PVVC_ERR_HIP;
-                hipGraphDestroy(g);
+                if (good) good = hip_ok(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+                if (g) hipGraphDestroy(g);
                 hipStreamDestroy(cs);
+                if (!good) {
+                    b->graph = nullptr;
+                    return PVVC_ERR_HIP;
+                }
                 b->graph_output = output;
                 std::copy(ptrs, ptrs + 3, b->graph_ptrs);
+                b->graph_waves = W;
+                b->graph_lds = lds;
+                b->graph_white = white;
             }
             auto ev = g_timing.next();
-            hipEventRecord(ev.first, g_stream);
-            hip_ok(hipGraphLaunch(b->graph, g_stream));
-            hipEventRecord(ev.second, g_stream);
-            return PVVC_OK;
+            hipEventRecord(ev.first, s);
+            const bool ok_launch = hip_ok(hipGraphLaunch(b->graph, s));
+            hipEventRecord(ev.second, s);
+            return ok_launch ? PVVC_OK : PVVC_ERR_HIP;
         }
         for (int f = 0; f < b->n_frames; ++f) {
             if (!b->frame_count[f]) continue;
             auto ev = g_timing.next();
             auto kev = g_ktiming.next();
-            hipEventRecord(ev.first, g_stream);
-            hipEventRecord(kev.first, g_stream);
-            if (W == 2) launch_frame<2>(b, f, sink, lds, g_stream);
-            else if (W == 4) launch_frame<4>(b, f, sink, lds, g_stream);
-            else launch_frame<1>(b, f, sink, lds, g_stream);
-            hipEventRecord(kev.second, g_stream);
-            hipEventRecord(ev.second, g_stream);
+            hipEventRecord(ev.first, s);
+            hipEventRecord(kev.first, s);
+            if (W == 2) launch_frame<2>(b, f, sink, lds, s);
+            else if (W == 4) launch_frame<4>(b, f, sink, lds, s);
+            else launch_frame<1>(b, f, sink, lds, s);
+            hipEventRecord(kev.second, s);
+            hipEventRecord(ev.second, s);
         }
         return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
     }
     if (mode != PVVC_MODE_PERSISTENT) return PVVC_ERR_INVALID_ARG;
-    uint64_t need = 0;
-    uint32_t grid_max = 0;
+    /* this call's workspace stride and slab: every workgroup of every
+     * segment gets `need` bytes at blockIdx * need */
+    uint64_t need = 256;
+    uint32_t grid_max = 1;
     for (auto& sg : b->segs) {
         need = std::max(need, sg.ws_need);
         grid_max = std::max(grid_max, persistent_grid(segment_tuning(d->tuning, d->auto_small, sg.count), sg.count));
     }
-    if (need * grid_max > d->slab_total || need > d->slab_bytes) {
+    const uint64_t slab_need = need * grid_max;
+    if (slab_need > d->slab_total) {
         if (d->slab) {
-            hipStreamSynchronize(g_stream);
+            hipStreamSynchronize(s);
             hipFree(d->slab);
             d->slab = nullptr;
+            d->slab_total = 0;
         }
-        if (!hip_ok(hipMalloc(&d->slab, std::max<uint64_t>(need * grid_max, 256)))) {
-            d->slab_total = d->slab_bytes = 0;
-            return PVVC_ERR_HIP;
-        }
-        d->slab_total = need * grid_max;
-        d->slab_bytes = need;
+        if (!hip_ok(hipMalloc(&d->slab, slab_need))) return PVVC_ERR_HIP;
+        d->slab_total = slab_need;
     }
+    d->slab_bytes = need;
     const uint64_t stage_need = 12ull * static_cast<uint64_t>(std::max<int64_t>(b->total_points, 1));
     if (stage_need > d->stage_bytes) {
         if (d->stage) {
-            hipStreamSynchronize(g_stream);
+            hipStreamSynchronize(s);
             hipFree(d->stage);
             d->stage = nullptr;
         }
@@ -2854,14 +2997,15 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
         d->stage_bytes = stage_need;
     }
     const BatchDev bd{b->items, b->order, b->defer, b->status};
-    const LaunchRes res{d->st.slots, d->slab, d->slab_bytes, d->stage, d->queue};
+    const LaunchRes res{d->st.slots, d->slab, need, d->slab_total, d->stage, d->queue};
     for (auto& sg : b->segs) {
         if (!sg.count) continue;
         const uint32_t id = ++d->launch_id == 0 ? ++d->launch_id : d->launch_id;
         auto ev = g_timing.next();
-        hipEventRecord(ev.first, g_stream);
-        launch_segment(bd, sg, sink, segment_tuning(d->tuning, d->auto_small, sg.count), res, id, g_stream);
-        hipEventRecord(ev.second, g_stream);
+        hipEventRecord(ev.first, s);
+        const int rc = launch_segment(bd, sg, sink, segment_tuning(d->tuning, d->auto_small, sg.count), res, id, s);
+        hipEventRecord(ev.second, s);
+        if (rc != PVVC_OK) return rc;
     }
     return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
 }
@@ -2880,7 +3024,7 @@ int decode_once(pvvc_decoder* d, const pvvc_frame* const* frames, int n_frames, 
     rc = pvvc_decode_batch(d, b, mode, output, xyz, rgb, points);
     std::vector<int> st(b->n_items, PVVC_OK);
     if (rc == PVVC_OK) rc = pvvc_batch_status(b, st.data(), static_cast<int>(st.size()));
-    else hipStreamSynchronize(g_stream);
+    else hipStreamSynchronize(b->last_stream);
     if (rc == PVVC_OK) {
         if (status) std::copy(st.begin(), st.begin() + std::min<size_t>(st.size(), status_cap), status);
         for (int s : st)
@@ -2927,10 +3071,10 @@ std::mutex g_enc_mu;
 
 uint64_t grow_size(uint64_t need) { return ((need + need / 4) + 0xFFFFF) & ~uint64_t(0xFFFFF); }
 
-bool enc_grow_dev(uint64_t need) {
+bool enc_grow_dev(uint64_t need, hipStream_t es) {
     if (need <= g_enc.dev_cap) return true;
     if (g_enc.dev) {
-        hipStreamSynchronize(g_stream);
+        hipStreamSynchronize(es);
         hipFree(g_enc.dev);
         g_enc.dev = nullptr;
         g_enc.dev_cap = 0;
@@ -2941,10 +3085,10 @@ bool enc_grow_dev(uint64_t need) {
     return true;
 }
 
-bool enc_grow_pinned(uint8_t*& p, uint64_t& cap_now, uint64_t need) {
+bool enc_grow_pinned(uint8_t*& p, uint64_t& cap_now, uint64_t need, hipStream_t es) {
     if (need <= cap_now) return true;
     if (p) {
-        hipStreamSynchronize(g_stream);
+        hipStreamSynchronize(es);
         hipHostFree(p);
         p = nullptr;
         cap_now = 0;
@@ -2979,6 +3123,7 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     if (!jobs || n_jobs <= 0) return PVVC_ERR_INVALID_ARG;
     rt_init_kernels();
     std::lock_guard<std::mutex> lk(g_enc_mu);
+    const hipStream_t es = default_stream(); /* one stream for the whole call */
     auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
     const uint64_t o_jobs = 0, o_res = al(sizeof(EncJobDev) * n_jobs), o_at = al(o_res + 8ull * n_jobs);
     const uint64_t in_begin = al(o_at + 8ull * n_jobs);
@@ -3005,7 +3150,7 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
         at += ws_cap[i];
     }
     const uint64_t total = at;
-    if (!enc_grow_dev(total) || !enc_grow_pinned(g_enc.pin, g_enc.pin_cap, in_end)) return PVVC_ERR_HIP;
+    if (!enc_grow_dev(total, es) || !enc_grow_pinned(g_enc.pin, g_enc.pin_cap, in_end, es)) return PVVC_ERR_HIP;
     if (!g_enc.e0 && !hip_ok(hipEventCreate(&g_enc.e0))) return PVVC_ERR_HIP;
     if (!g_enc.e1 && !hip_ok(hipEventCreate(&g_enc.e1))) return PVVC_ERR_HIP;
     uint8_t* const dev = g_enc.dev;
@@ -3032,7 +3177,7 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
             jd[r] = d;
         }
     }
-    bool good = hip_ok(hipMemcpyAsync(dev + o_jobs, h + o_jobs, o_res, hipMemcpyHostToDevice, g_stream));
+    bool good = hip_ok(hipMemcpyAsync(dev + o_jobs, h + o_jobs, o_res, hipMemcpyHostToDevice, es));
     /* chunks of consecutive jobs of about 4 MiB of input each (at most 64) */
     const uint64_t in_bytes = in_end - in_begin;
     const int n_chunks = static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>({64, uint64_t(n_jobs),
@@ -3065,7 +3210,7 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     for (int c = 0; c < n_chunks; ++c) {
         while (!done[c].load(std::memory_order_acquire)) std::this_thread::yield();
         const uint64_t b = start(c), e = start(c + 1);
-        if (good && e > b) good = hip_ok(hipMemcpyAsync(dev + b, h + b, e - b, hipMemcpyHostToDevice, g_stream));
+        if (good && e > b) good = hip_ok(hipMemcpyAsync(dev + b, h + b, e - b, hipMemcpyHostToDevice, es));
     }
     for (auto& th : pool) th.join();
     /* 8 KiB per stream lets 16 encode workgroups share a CU, the most its
@@ -3074,15 +3219,15 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(8 * 1024));
     std::vector<int64_t> res(n_jobs, 0);
     if (good) {
-        hipEventRecord(g_enc.e0, g_stream);
+        hipEventRecord(g_enc.e0, es);
         const uint32_t wave1 = static_cast<uint32_t>(std::max(1, devinfo().cu) * 16);
-        hipLaunchKernelGGL(encode_kernel, dim3(n_jobs), dim3(64), lds, g_stream,
+        hipLaunchKernelGGL(encode_kernel, dim3(n_jobs), dim3(64), lds, es,
                            reinterpret_cast<const EncJobDev*>(dev + o_jobs), static_cast<uint32_t>(n_jobs), lds,
                            reinterpret_cast<int64_t*>(dev + o_res), wave1);
-        hipEventRecord(g_enc.e1, g_stream);
+        hipEventRecord(g_enc.e1, es);
         good = hip_ok(hipGetLastError()) &&
-               hip_ok(hipMemcpyAsync(res.data(), dev + o_res, 8ull * n_jobs, hipMemcpyDeviceToHost, g_stream)) &&
-               hip_ok(hipStreamSynchronize(g_stream));
+               hip_ok(hipMemcpyAsync(res.data(), dev + o_res, 8ull * n_jobs, hipMemcpyDeviceToHost, es)) &&
+               hip_ok(hipStreamSynchronize(es));
     }
     if (good && kernel_ms) {
         float ms = 0;
@@ -3103,16 +3248,16 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
             good = hip_ok(hipMalloc(&spill, packed_bytes));
             packed = spill;
         }
-        good = good && enc_grow_pinned(g_enc.pin_out, g_enc.pin_out_cap, packed_bytes) &&
-               hip_ok(hipMemcpyAsync(dev + o_at, pos.data(), 8ull * n_jobs, hipMemcpyHostToDevice, g_stream));
+        good = good && enc_grow_pinned(g_enc.pin_out, g_enc.pin_out_cap, packed_bytes, es) &&
+               hip_ok(hipMemcpyAsync(dev + o_at, pos.data(), 8ull * n_jobs, hipMemcpyHostToDevice, es));
         if (good) {
-            hipLaunchKernelGGL(gather_kernel, dim3(n_jobs), dim3(256), 0, g_stream,
+            hipLaunchKernelGGL(gather_kernel, dim3(n_jobs), dim3(256), 0, es,
                                reinterpret_cast<const EncJobDev*>(dev + o_jobs),
                                reinterpret_cast<const int64_t*>(dev + o_res),
                                reinterpret_cast<const uint64_t*>(dev + o_at), static_cast<uint32_t>(n_jobs), packed);
             good = hip_ok(hipGetLastError()) &&
-                   hip_ok(hipMemcpyAsync(g_enc.pin_out, packed, packed_bytes, hipMemcpyDeviceToHost, g_stream)) &&
-                   hip_ok(hipStreamSynchronize(g_stream));
+                   hip_ok(hipMemcpyAsync(g_enc.pin_out, packed, packed_bytes, hipMemcpyDeviceToHost, es)) &&
+                   hip_ok(hipStreamSynchronize(es));
         }
         if (good) {
             const uint8_t* hp = g_enc.pin_out;
@@ -3135,7 +3280,7 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
         jobs[i].status = !good ? PVVC_ERR_HIP : (res[i] > 0 ? PVVC_OK : static_cast<int>(-res[i]));
     }
     if (spill) {
-        hipStreamSynchronize(g_stream);
+        hipStreamSynchronize(es);
         hipFree(spill);
     }
     return good ? PVVC_OK : PVVC_ERR_HIP;
@@ -3144,7 +3289,7 @@ int pvvc_encode_colors(pvvc_color_job* jobs, int n_jobs, double* kernel_ms) {
 /* Frees the colour encoder's kept buffers (they are made again on the next call). */
 void pvvc_encoder_release(void) {
     std::lock_guard<std::mutex> lk(g_enc_mu);
-    if (g_stream) hipStreamSynchronize(g_stream);
+    hipStreamSynchronize(default_stream());
     if (g_enc.dev) hipFree(g_enc.dev);
     if (g_enc.pin) hipHostFree(g_enc.pin);
     if (g_enc.pin_out) hipHostFree(g_enc.pin_out);
@@ -3180,11 +3325,11 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
         }
         d->tuning = t;
         int rc = pvvc_decode_batch(d, b, mode, output, xyz, rgb, points);
-        if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(g_stream))) rc = PVVC_ERR_HIP;
+        if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(dec_stream(d)))) rc = PVVC_ERR_HIP;
         std::vector<double> v;
         for (int r = 0; r < reps && rc == PVVC_OK; ++r) {
             rc = pvvc_decode_batch(d, b, mode, output, xyz, rgb, points);
-            if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(g_stream))) rc = PVVC_ERR_HIP;
+            if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(dec_stream(d)))) rc = PVVC_ERR_HIP;
             v.push_back(pvvc_last_kernel_ms());
         }
         if (rc != PVVC_OK) return rc;
@@ -3228,7 +3373,7 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
     if (best_ms) *best_ms = cur_ms;
     /* leave the outputs as the chosen configuration writes them */
     rc = pvvc_decode_batch(d, b, mode, output, xyz, rgb, points);
-    if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(g_stream))) rc = PVVC_ERR_HIP;
+    if (rc == PVVC_OK && !hip_ok(hipStreamSynchronize(dec_stream(d)))) rc = PVVC_ERR_HIP;
     return rc;
 }
 

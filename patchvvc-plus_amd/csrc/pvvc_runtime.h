@@ -60,7 +60,8 @@ struct BatchDev {
 struct LaunchRes {
     SlotDev* slots;
     uint8_t* slab;
-    uint64_t slab_bytes;   /* per workgroup */
+    uint64_t slab_bytes;   /* per workgroup: the stride between workgroups' workspaces */
+    uint64_t slab_total;   /* bytes of the whole slab */
     float* stage;
     uint32_t* queue;
 };
@@ -72,8 +73,10 @@ uint32_t persistent_grid(const pvvc_tuning& t, uint32_t count);
  * 32 KiB of LDS and 4 workgroups per CU. */
 pvvc_tuning segment_tuning(const pvvc_tuning& t, bool auto_small, uint32_t count);
 /* Enqueues one segment: persistent_kernel (its last workgroup re-arms the
- * queue) and the finish pass. */
-void launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, const pvvc_tuning& t,
+ * queue) and the finish pass.  PVVC_ERR_CAPACITY, with nothing enqueued, when
+ * the segment's workspace bound exceeds the slab stride or the grid's
+ * workspaces exceed the slab. */
+int launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, const pvvc_tuning& t,
                     const LaunchRes& r, uint32_t launch_id, hipStream_t s);
 /* Records a HIP failure in the library's sticky error; true on success. */
 bool rt_hip_ok(hipError_t e);

@@ -129,6 +129,7 @@ struct Lane {
     size_t status_cap = 0;
     uint8_t* slab = nullptr;
     uint64_t slab_bytes = 0;       /* per workgroup */
+    uint64_t slab_total = 0;
     float* stage = nullptr;
     uint32_t* queue = nullptr;
     bool busy = false;
@@ -400,7 +401,7 @@ void scheduler(pvvc_server* sv) {
                           reinterpret_cast<const uint32_t*>(ln.dev + plan.o_order),
                           reinterpret_cast<const uint32_t*>(ln.dev + plan.o_defer),
                           reinterpret_cast<int*>(ln.dev + plan.o_status)};
-        const LaunchRes res{sv->slots, ln.slab, ln.slab_bytes, ln.stage, ln.queue};
+        const LaunchRes res{sv->slots, ln.slab, ln.slab_bytes, ln.slab_total, ln.stage, ln.queue};
         const Sink sink{sv->pool_xyz, sv->pool_rgb, nullptr, PVVC_OUT_NATIVE, 0};
         rt_timing_reset();
         for (const Segment& sg : plan.segs) {
@@ -409,7 +410,7 @@ void scheduler(pvvc_server* sv) {
                 std::lock_guard<std::mutex> g(sv->mu);
                 id = ++sv->launch_id;
             }
-            launch_segment(bd, sg, sink, tun, res, id, ln.s);
+            if (launch_segment(bd, sg, sink, tun, res, id, ln.s) != PVVC_OK) good = false;
         }
         good &= ok(hipGetLastError());
         good &= ok(hipMemcpyAsync(ln.status_host, ln.dev + plan.o_status, 4ull * plan.n_items, hipMemcpyDeviceToHost,
@@ -538,6 +539,7 @@ int pvvc_server_create(const pvvc_server_config* cfg_in, pvvc_server** out) {
                ok(hipMalloc(&ln.slab, need * grid)) && ok(hipMalloc(&ln.stage, 12ull * cfg.max_batch_points)) &&
                ok(hipMalloc(&ln.queue, 64));
         ln.slab_bytes = need;
+        ln.slab_total = need * grid;
     }
     /* first launches load the code object and first-use buffers grow: pay
      * both here rather than in the first frames' latency */

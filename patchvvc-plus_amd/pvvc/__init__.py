@@ -25,7 +25,7 @@ __all__ = [
     "PvvcError", "StagedFrame", "launch_dropin", "synchronize", "last_kernel_ms", "last_launch_times",
     "STATUS", "Frame", "DeviceBuffer", "State", "Decoder", "Batch", "decode_dropin", "synth_stream",
     "SYNTH_PRESETS", "Points", "hip_lib", "enc_lib", "MODE_PER_FRAME", "MODE_PERSISTENT", "MODE_GRAPH",
-    "OUT_NATIVE", "OUT_POINTS", "device_info", "encode_colors_packed",
+    "OUT_NATIVE", "OUT_POINTS", "device_info", "encode_colors_packed", "Stream",
 ]
 
 STATUS = {
@@ -225,6 +225,42 @@ class State:
     def status(self, patch: int) -> int:
         return int(hip_lib().pvvc_state_patch_status(C.c_void_p(self.ptr), patch))
 
+    def set_stream(self, stream: "Stream | None") -> None:
+        """This state's own stream for launch_cudaProcess (None: the null stream)."""
+        _check(hip_lib().pvvc_state_set_stream(C.c_void_p(self.ptr), C.c_void_p(stream.handle if stream else None)),
+               "pvvc_state_set_stream")
+
+    def set_dropin_tuning(self, waves: int, lds_bytes: int) -> None:
+        _check(hip_lib().pvvc_state_set_dropin_tuning(C.c_void_p(self.ptr), waves, lds_bytes),
+               "pvvc_state_set_dropin_tuning")
+
+    def synchronize(self) -> int:
+        """Waits for the state's stream; returns the first failed slice's status since the last read."""
+        return int(hip_lib().pvvc_state_synchronize(C.c_void_p(self.ptr)))
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority (pvvc_stream_create)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(hip_lib().pvvc_stream_create(C.byref(h)), "pvvc_stream_create")
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        _check(hip_lib().pvvc_stream_synchronize(C.c_void_p(self.handle)), "pvvc_stream_synchronize")
+
+    def close(self) -> None:
+        if self.handle:
+            hip_lib().pvvc_stream_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 class StagedFrame:
     """Device arrays of one drop-in call (pvvc_dropin_stage)."""
@@ -299,7 +335,7 @@ def decode_dropin(frame: Frame, state: State, out: DeviceBuffer | None = None) -
     if own:
         out = DeviceBuffer(max(1, n) * 24)
     launch_dropin(staged, state, out, frame.timestamp)
-    synchronize()
+    state.synchronize()  # the state's stream; launch errors go to this thread's pvvc_last_error
     res = out.read(POINTS_DTYPE, n)
     staged.release()
     if own:
@@ -373,6 +409,15 @@ class Decoder:
     def set_tuning(self, waves_per_patch=0, lds_bytes=0, wgs_per_cu=0, queue_chunk=0, priority_items=0) -> None:
         t = Tuning(waves_per_patch, lds_bytes, wgs_per_cu, queue_chunk, priority_items)
         _check(hip_lib().pvvc_decoder_set_tuning(self.handle, C.byref(t)), "set_tuning")
+
+    def set_stream(self, stream: "Stream | None") -> None:
+        """The decoder's own stream for its decodes (None: the null stream)."""
+        _check(hip_lib().pvvc_decoder_set_stream(self.handle, C.c_void_p(stream.handle if stream else None)),
+               "pvvc_decoder_set_stream")
+
+    def set_white_small_patches(self, enable: int) -> None:
+        _check(hip_lib().pvvc_decoder_set_white_small_patches(self.handle, int(enable)),
+               "pvvc_decoder_set_white_small_patches")
 
     def tuning(self) -> dict:
         t = Tuning()
@@ -655,16 +700,24 @@ class Server:
 # ---------------------------------------------------------------------------
 # Synthetic streams (stream writer)
 # ---------------------------------------------------------------------------
+# patch_min / patch_max / patch_mu / patch_sigma drive the recursive median
+# split: a group stops splitting at a lognormal target clamped to
+# [patch_min, patch_max], so final patches fall between about half the
+# target and the target.  Measured, first frame, seed 7 (5, 6 for c2/c4):
+#   c0 121 856 points, 123 patches of 669-1993 (median 996)
+#   c1 816 807 points, 866 patches of 207-2413
+#   c2 3.30M points, ~2 500 patches of 52-18 332
+#   c4 5.49M points, ~3 400 patches of 22-46 760
 SYNTH_PRESETS = {
-    # ~120k voxels/frame, 9-bit, ~1000-point patches, GoP 8, qp 8
+    # ~120k voxels/frame, 9-bit, ~1000-point patches (~120 per frame), GoP 8, qp 8
     "c0": dict(grid_bits=9, gop=8, qp_i=8, qp_p=8, n_humanoids=1, jitter=0, patch_min=1100, patch_max=2000,
                scale=0.85, shell=0.9, patch_mu=float(np.log(1400)), patch_sigma=0.2),
     # ~800k points/frame, 10-bit, ~800 patches of lognormal 200-3000 points, GoP 8, qp 4, +-1 voxel jitter
     "c1": dict(grid_bits=10, gop=8, qp_i=4, qp_p=4, n_humanoids=1, jitter=1, patch_min=400, patch_max=3000,
                scale=0.85, shell=1.5, patch_mu=float(np.log(1500)), patch_sigma=0.7),
     # ~3M points/frame, 11-bit, heavy-tailed lognormal 50-20000, GoP 16
-    "c2": dict(grid_bits=11, gop=16, qp_i=4, qp_p=4, n_humanoids=1, jitter=1, patch_min=100, patch_max=20000,
-               scale=0.85, shell=1.5, patch_mu=float(np.log(3000)), patch_sigma=1.2),
+    "c2": dict(grid_bits=11, gop=16, qp_i=4, qp_p=4, n_humanoids=1, jitter=1, patch_min=80, patch_max=20000,
+               scale=0.85, shell=1.5, patch_mu=float(np.log(1500)), patch_sigma=1.6),
     # serving densities: 100k / 9-bit, 400k / 10-bit, 1M / 10-bit
     "c3_100k": dict(grid_bits=9, gop=8, qp_i=4, qp_p=4, n_humanoids=1, jitter=0, patch_min=400, patch_max=3000,
                     scale=0.85, shell=0.75, patch_mu=float(np.log(1500)), patch_sigma=0.7),
@@ -673,8 +726,8 @@ SYNTH_PRESETS = {
     "c3_1m": dict(grid_bits=10, gop=8, qp_i=4, qp_p=4, n_humanoids=1, jitter=1, patch_min=400, patch_max=3000,
                   scale=0.85, shell=1.85, patch_mu=float(np.log(1500)), patch_sigma=0.7),
     # 8 humanoids, 12-bit, ~5M points/frame, GoP 32, qp 1, patches 20-50000
-    "c4": dict(grid_bits=12, gop=32, qp_i=1, qp_p=1, n_humanoids=8, jitter=1, patch_min=40, patch_max=50000,
-               scale=0.85, shell=0.7, patch_mu=float(np.log(4000)), patch_sigma=1.4),
+    "c4": dict(grid_bits=12, gop=32, qp_i=1, qp_p=1, n_humanoids=8, jitter=1, patch_min=20, patch_max=50000,
+               scale=0.85, shell=0.7, patch_mu=float(np.log(1500)), patch_sigma=2.0),
 }
 
 

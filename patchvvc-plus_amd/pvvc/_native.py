@@ -105,6 +105,15 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_state_alloc", c_vp, C.c_int, C.c_int)
     _sig(L, "pvvc_state_free", None, c_vp)
     _sig(L, "pvvc_state_patch_status", C.c_int, c_vp, C.c_int)
+    _sig(L, "pvvc_state_set_stream", C.c_int, c_vp, c_vp)
+    _sig(L, "pvvc_state_set_dropin_tuning", C.c_int, c_vp, C.c_int, C.c_int)
+    _sig(L, "pvvc_state_set_white_small_patches", C.c_int, c_vp, C.c_int)
+    _sig(L, "pvvc_state_synchronize", C.c_int, c_vp)
+    _sig(L, "pvvc_decoder_set_stream", C.c_int, c_vp, c_vp)
+    _sig(L, "pvvc_decoder_set_white_small_patches", C.c_int, c_vp, C.c_int)
+    _sig(L, "pvvc_stream_create", C.c_int, C.POINTER(c_vp))
+    _sig(L, "pvvc_stream_destroy", C.c_int, c_vp)
+    _sig(L, "pvvc_stream_synchronize", C.c_int, c_vp)
     _sig(L, "launch_cudaProcess", None, C.c_int, C.c_int, c_vp, C.c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
          c_vp, c_vp, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_set_stream", C.c_int, c_vp)

@@ -1,5 +1,5 @@
 """GPU parity at the large configurations: c2 (11-bit, ~3M points/frame,
-patches 100-20000) and c4 (12-bit, 8 humanoids, ~5M points/frame, qp 1,
+patches 50-20000) and c4 (12-bit, 8 humanoids, ~5M points/frame, qp 1,
 patches up to ~50000).  Large patches exceed the LDS budget and stream
 their workspace through global memory; deep octrees and qp 1 give long RLGR
 runs.  Two frames (intra + predictive) per configuration."""

@@ -150,3 +150,30 @@ def test_poisson_schedule_properties():
         per[s] = k
     rate = len(ev) / (16 * 20.0)
     assert 27.0 < rate < 33.0
+
+
+def test_last_error_is_per_thread(pvvc):
+    """A failure on one host thread is that thread's pvvc_last_error alone:
+    launch_cudaProcess with an unknown state records INVALID_ARG (before any
+    device call) on the calling thread, and the main thread reads 0."""
+    import threading
+    L = pvvc.hip_lib()
+    L.pvvc_last_error()
+    seen = {}
+
+    def worker():
+        L.launch_cudaProcess(1, 1, None, 0, None, None, None, None, None, None, None, None, None, None,
+                             C_void(0xdead0), 3)
+        seen["worker"] = L.pvvc_last_error()
+        seen["worker_again"] = L.pvvc_last_error()
+
+    t = threading.Thread(target=worker)
+    t.start()
+    t.join()
+    assert seen == {"worker": 8, "worker_again": 0}
+    assert L.pvvc_last_error() == 0
+
+
+def C_void(v):
+    import ctypes as C
+    return C.c_void_p(v)
