@@ -272,6 +272,23 @@ int pvvc_decode_gop(pvvc_decoder* dec, const pvvc_frame* const* frames, int n_fr
 /* Resets every patch state slot (drops GoP state). */
 int pvvc_decoder_reset(pvvc_decoder* dec);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The PVVC_MODE_PERSISTENT decode of pvvc_decode_batch,
+ * identical output, with no allocation of any kind: the queue counters, the
+ * stage of deferred predictive slices and the per-workgroup workspaces come
+ * from one library-owned device array of 256 MiB per GPU (held until the
+ * process ends), and the launch's workgroups are cut to the workspaces that
+ * fit there.  PVVC_ERR_CAPACITY, with nothing enqueued, when the batch's
+ * stage (12 B per point of the batch) and one workspace exceed it (about
+ * 21 M points per batch).  The decoder and the batch are the caller's, made
+ * beforehand (pvvc_decoder_create / pvvc_batch_create allocate).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads, other decoders) race, so the caller serialises
+ * them.  Asynchronous on the decoder's stream; no kernel times are recorded. */
+int pvvc_decode_batch_noalloc(pvvc_decoder* dec, pvvc_batch* batch, int output, float* xyz, uint8_t* rgb,
+                              Points* points);
+
 /* Autotuner: picks waves per slice (1/2/4), the LDS budget per slice
  * (4-64 KiB) and, for the persistent mode, workgroups per CU and the queue
  * chunk by coordinate descent, timing `reps` decodes of `batch` per

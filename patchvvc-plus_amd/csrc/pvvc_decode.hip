@@ -1864,6 +1864,16 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncJobDev* __restrict_
 
 }  // namespace pvvc
 
+/* Device scratch of pvvc_decode_batch_noalloc: the queue counters, the stage
+ * of deferred predictive items and the per-workgroup workspaces of a
+ * persistent decode, for callers that cannot allocate.  Each GPU the library
+ * runs on holds its own copy until the process ends.  Host code never passes
+ * this name to a kernel (on the host it is only a stand-in for the symbol):
+ * it uses the device address hipGetSymbolAddress returns for the current
+ * device (scratch_base()). */
+constexpr uint64_t kScratchBytes = 256ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
 /* ======================================================================== */
 /* Host                                                                      */
 /* ======================================================================== */
@@ -1915,6 +1925,25 @@ DeviceInfo& devinfo() {
         }
     });
     return d;
+}
+
+/* Device address of `scratch` on each device, looked up once per device with
+ * that device current (pvvc_decoder_create, else the first
+ * pvvc_decode_batch_noalloc on the device). */
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_at[kMaxDevices] = {};
+
+uint8_t* scratch_base() {
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev)) || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        void* p = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) || !p) return nullptr;
+        g_scratch_at[dev] = static_cast<uint8_t*>(p);
+    }
+    return g_scratch_at[dev];
 }
 
 /* Launch-time records of the last decode call: event pairs per main launch. */
@@ -2289,11 +2318,16 @@ void launch_frame(const pvvc_batch* b, int f, const Sink& sink, uint32_t lds, hi
 template <int W>
 void launch_persistent_w(const BatchDev& b, const Segment& sg, const Sink& sink, uint32_t lds, uint32_t grid,
                          uint32_t chunk, uint32_t prio, const LaunchRes& r, uint32_t launch_id, hipStream_t s) {
-    auto ev = g_ktiming.next();
-    hipEventRecord(ev.first, s);
+    /* untimed launches (grid_cap set: the scratch-backed entry point) record no events */
+    const bool timed = r.grid_cap == 0;
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    if (timed) {
+        ev = g_ktiming.next();
+        hipEventRecord(ev.first, s);
+    }
     hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b.items, b.order + sg.begin, sg.count,
                        r.queue, r.slots, sink, lds, r.slab, r.slab_bytes, launch_id, b.status, r.stage, chunk, prio);
-    hipEventRecord(ev.second, s);
+    if (timed) hipEventRecord(ev.second, s);
     if (sg.dcount)
         hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, b.items, b.defer + sg.dbegin, sg.dcount,
                            r.slots, sink, r.stage, b.status);
@@ -2375,7 +2409,8 @@ int launch_segment(const BatchDev& b, const Segment& sg, const Sink& sink, const
                    uint32_t launch_id, hipStream_t s) {
     if (!sg.count) return PVVC_OK;
     const uint32_t lds = static_cast<uint32_t>(clamp_lds(t.lds_bytes));
-    const uint32_t grid = persistent_grid(t, sg.count);
+    uint32_t grid = persistent_grid(t, sg.count);
+    if (r.grid_cap) grid = std::min(grid, r.grid_cap);
     /* workgroup g uses slab bytes [g * stride, g * stride + stride) */
     if (sg.ws_need > r.slab_bytes || static_cast<uint64_t>(grid) * r.slab_bytes > r.slab_total) {
         set_error(PVVC_ERR_CAPACITY);
@@ -2771,6 +2806,15 @@ int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_
         return PVVC_ERR_HIP;
     }
     d->intra_geo.assign(static_cast<size_t>(max_patches), 0u);
+    /* set-up of the allocation-free entry point: kernel attributes, the Haar
+     * table and this device's scratch address */
+    rt_init_kernels();
+    if (!scratch_base()) {
+        state_destroy(d->st);
+        hipFree(d->queue);
+        delete d;
+        return PVVC_ERR_HIP;
+    }
     *out = d;
     return PVVC_OK;
 }
@@ -2997,7 +3041,7 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
         d->stage_bytes = stage_need;
     }
     const BatchDev bd{b->items, b->order, b->defer, b->status};
-    const LaunchRes res{d->st.slots, d->slab, need, d->slab_total, d->stage, d->queue};
+    const LaunchRes res{d->st.slots, d->slab, need, d->slab_total, d->stage, d->queue, 0u};
     for (auto& sg : b->segs) {
         if (!sg.count) continue;
         const uint32_t id = ++d->launch_id == 0 ? ++d->launch_id : d->launch_id;
@@ -3046,6 +3090,50 @@ int pvvc_decode_frame(pvvc_decoder* dec, const pvvc_frame* frame, int output, fl
 int pvvc_decode_gop(pvvc_decoder* dec, const pvvc_frame* const* frames, int n_frames, int output, float* xyz,
                     uint8_t* rgb, Points* points, int* status, int status_cap) {
     return decode_once(dec, frames, n_frames, PVVC_MODE_PERSISTENT, output, xyz, rgb, points, status, status_cap);
+}
+
+/* The persistent decode of pvvc_decode_batch with every device buffer it
+ * would allocate taken from `scratch`: [queue counters 256 B][stage: 12 B per
+ * point of the batch][workspaces: one per workgroup at this call's stride].
+ * The grid is cut to the workspaces that fit.  Enqueues one counter-reset
+ * kernel and the segments' launches; no allocation, no memset, no events. */
+int pvvc_decode_batch_noalloc(pvvc_decoder* d, pvvc_batch* b, int output, float* xyz, uint8_t* rgb, Points* points) {
+    if (!d || !b || b->dec != d) return PVVC_ERR_INVALID_ARG;
+    if (output == PVVC_OUT_NATIVE && (!xyz || !rgb)) return PVVC_ERR_INVALID_ARG;
+    if (output == PVVC_OUT_POINTS && !points) return PVVC_ERR_INVALID_ARG;
+    if (output != PVVC_OUT_NATIVE && output != PVVC_OUT_POINTS) return PVVC_ERR_INVALID_ARG;
+    uint8_t* base = scratch_base();
+    if (!base) return PVVC_ERR_HIP;
+    auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+    const uint64_t o_stage = 256;
+    const uint64_t stage_bytes = al(12ull * static_cast<uint64_t>(std::max<int64_t>(b->total_points, 1)));
+    uint64_t need = 256;
+    for (auto& sg : b->segs) need = std::max(need, sg.ws_need);
+    if (o_stage + stage_bytes + need > kScratchBytes) {
+        set_error(PVVC_ERR_CAPACITY);
+        return PVVC_ERR_CAPACITY; /* the batch's stage or one workspace exceeds the scratch */
+    }
+    const uint64_t o_slab = o_stage + stage_bytes;
+    const uint64_t slab_total = kScratchBytes - o_slab;
+    const uint32_t grid_cap = static_cast<uint32_t>(std::min<uint64_t>(slab_total / need, 0xffffffffu));
+    const hipStream_t s = dec_stream(d);
+    b->last_stream = s;
+    const Sink sink{xyz, rgb, points, output, dec_white_small(d)};
+    uint32_t* queue = reinterpret_cast<uint32_t*>(base);
+    /* the counters start at zero: a kernel, not a memset, so that the call
+     * can be captured into a graph */
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, queue);
+    if (!hip_ok(hipGetLastError())) return PVVC_ERR_HIP;
+    const BatchDev bd{b->items, b->order, b->defer, b->status};
+    const LaunchRes res{d->st.slots, base + o_slab, need, slab_total, reinterpret_cast<float*>(base + o_stage),
+                        queue, grid_cap};
+    for (auto& sg : b->segs) {
+        if (!sg.count) continue;
+        const uint32_t id = ++d->launch_id == 0 ? ++d->launch_id : d->launch_id;
+        const int rc = launch_segment(bd, sg, sink, segment_tuning(d->tuning, d->auto_small, sg.count), res, id, s);
+        if (rc != PVVC_OK) return rc;
+    }
+    return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
 }
 
 }  // extern "C"

@@ -64,6 +64,7 @@ struct LaunchRes {
     uint64_t slab_total;   /* bytes of the whole slab */
     float* stage;
     uint32_t* queue;
+    uint32_t grid_cap;     /* at most this many workgroups (0: no cap); a capped launch is untimed */
 };
 
 /* Workgroups of a persistent launch of `count` items. */

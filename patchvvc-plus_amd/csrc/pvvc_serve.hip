@@ -401,7 +401,7 @@ void scheduler(pvvc_server* sv) {
                           reinterpret_cast<const uint32_t*>(ln.dev + plan.o_order),
                           reinterpret_cast<const uint32_t*>(ln.dev + plan.o_defer),
                           reinterpret_cast<int*>(ln.dev + plan.o_status)};
-        const LaunchRes res{sv->slots, ln.slab, ln.slab_bytes, ln.slab_total, ln.stage, ln.queue};
+        const LaunchRes res{sv->slots, ln.slab, ln.slab_bytes, ln.slab_total, ln.stage, ln.queue, 0u};
         const Sink sink{sv->pool_xyz, sv->pool_rgb, nullptr, PVVC_OUT_NATIVE, 0};
         rt_timing_reset();
         for (const Segment& sg : plan.segs) {

@@ -143,6 +143,7 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_batch_state_bytes", C.c_int64, c_vp)
     _sig(L, "pvvc_batch_status", C.c_int, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_decode_batch", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp)
+    _sig(L, "pvvc_decode_batch_noalloc", C.c_int, c_vp, c_vp, C.c_int, c_vp, c_vp, c_vp)
     _sig(L, "pvvc_decode_frame", C.c_int, c_vp, c_vp, C.c_int, c_vp, c_vp, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_decode_gop", C.c_int, c_vp, c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp, C.c_int)
     _sig(L, "pvvc_encode_colors", C.c_int, c_vp, C.c_int, C.POINTER(C.c_double))
