@@ -1,9 +1,10 @@
 """Benchmark of the MI355X per-patch GoP decoder.
 
 Workload (default): the config-1 stream -- synthetic humanoid, 10-bit grid,
-~800k points/frame, ~800 patches of lognormal 200-3000 points, GoP 8,
-quantisation step 4 -- generated on the host, packed and resident in HBM
-before timing.  One step = decode of one whole GoP (8 frames).
+~800k points/frame, ~860 patches of lognormal 200-3000 points, GoP 8,
+quantisation step 4 -- four consecutive GoPs of one seeded stream, generated
+on the host, packed and resident in HBM before timing.  One step = decode of
+one whole GoP (8 frames); the steps cycle over the four GoPs.
 
 Reported: `value` = decoded Mpts/s of the batched persistent decode (whole
 GoP per launch); beside it the per-frame drop-in decode (launch_cudaProcess,
@@ -27,6 +28,8 @@ sys.path.insert(0, str(REPO / "patchvvc-plus_amd"))
 sys.path.insert(0, str(REPO))
 
 HBM_PEAK_GBS = 8000.0
+ENGINE_CLOCK_HZ = 2.4e9   # MI355X peak engine clock
+XCDS = 8                  # MI355X compute dies; GRBM_GUI_ACTIVE reads the sum of their cycles
 B_OUT_NATIVE = 15   # xyz f32 x3 + RGB8
 B_OUT_POINTS = 24   # reference Points AoS
 
@@ -39,23 +42,27 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else float("nan")
 
 
-def _pmc_kib(csv_path: Path, kernel: str) -> float | None:
-    """Mean counter value (KiB per dispatch) of the dispatches of `kernel`."""
+def _pmc_means(csv_path: Path, kernel: str) -> dict:
+    """Mean value per counter (over the dispatches of `kernel`) of a
+    rocprofv3 counter-collection CSV."""
     import csv
-    vals = []
+    vals: dict[str, list[float]] = {}
     with open(csv_path, newline="") as fh:
         for row in csv.DictReader(fh):
             if kernel in row["Kernel_Name"]:
-                vals.append(float(row["Counter_Value"]))
-    return float(np.mean(vals)) if vals else None
+                vals.setdefault(row.get("Counter_Name", "?"), []).append(float(row["Counter_Value"]))
+    return {k: float(np.mean(v)) for k, v in vals.items()}
 
 
-def measure_traffic(args) -> dict | None:
-    """HBM bytes per persistent_kernel launch from two rocprofv3 --pmc passes
-    (FETCH_SIZE, then WRITE_SIZE) of this script's batched leg, each scaled by
-    what the same counter reads for a device copy of 256 MiB.  Runs as child
-    processes before this process touches the GPU; None when rocprofv3 is
-    absent or a pass fails."""
+def measure_counters(args) -> dict | None:
+    """Per persistent_kernel launch, from rocprofv3 --pmc passes of this
+    script's batched leg (each a child process, before this process touches
+    the GPU):
+      * HBM bytes: FETCH_SIZE, then WRITE_SIZE, each scaled by what the same
+        counter reads for a device copy of 256 MiB;
+      * scalar issue: SQ_INSTS_SALU (instructions of the whole launch) with
+        GRBM_GUI_ACTIVE (GPU cycles of the launch).
+    None when rocprofv3 is absent or a pass fails."""
     import shutil
     import subprocess
     import tempfile
@@ -63,71 +70,84 @@ def measure_traffic(args) -> dict | None:
     if not prof:
         return None
     me = [sys.executable, "-u", str(Path(__file__).resolve())]
-    leg = me + ["--only", "batched", "--steps", "2", "--warmup", "1", "--no-cpu", "--no-traffic", "--no-encode",
-                "--no-quality", "--preset", args.preset, "--seed", str(args.seed)]
+    leg = me + ["--only", "batched", "--steps", str(args.gops), "--warmup", "1", "--gops", str(args.gops),
+                "--no-cpu", "--no-traffic", "--no-encode", "--no-quality", "--no-c2", "--preset", args.preset,
+                "--seed", str(args.seed)]
     for k in ("waves", "lds", "wgs_per_cu", "queue_chunk", "priority_items"):
         v = getattr(args, k)
         if v:
             leg += ["--" + k.replace("_", "-"), str(v)]
     copy_mib = 256
+    copy = me + ["--calibrate-copy", str(copy_mib)]
     res = {}
     with tempfile.TemporaryDirectory() as d:
-        for tag, counter, cmd, kern in (("fetch", "FETCH_SIZE", leg, "persistent_kernel"),
-                                         ("write", "WRITE_SIZE", leg, "persistent_kernel"),
-                                         ("copy_fetch", "FETCH_SIZE", me + ["--calibrate-copy", str(copy_mib)],
-                                          "copyBuffer"),
-                                         ("copy_write", "WRITE_SIZE", me + ["--calibrate-copy", str(copy_mib)],
-                                          "copyBuffer")):
+        for tag, counters, cmd, kern in (("fetch", ["FETCH_SIZE"], leg, "persistent_kernel"),
+                                          ("write", ["WRITE_SIZE"], leg, "persistent_kernel"),
+                                          ("salu", ["SQ_INSTS_SALU", "GRBM_GUI_ACTIVE"], leg, "persistent_kernel"),
+                                          ("copy_fetch", ["FETCH_SIZE"], copy, "copyBuffer"),
+                                          ("copy_write", ["WRITE_SIZE"], copy, "copyBuffer")):
             out = Path(d) / tag
-            full = [prof, "--pmc", counter, "-f", "csv", "-d", str(out), "-o", "p", "--"] + cmd
-            log("traffic pass:", tag, counter)
+            full = [prof, "--pmc", *counters, "-f", "csv", "-d", str(out), "-o", "p", "--"] + cmd
+            log("counter pass:", tag, " ".join(counters))
             try:
-                r = subprocess.run(full, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120)
+                r = subprocess.run(full, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=180)
             except subprocess.TimeoutExpired:
-                log("traffic pass timed out:", tag)
+                log("counter pass timed out:", tag)
                 return None
             csvs = list(out.rglob("*counter_collection.csv"))
             if r.returncode != 0 or not csvs:
-                log("traffic pass failed:", tag, r.returncode, r.stdout[-2000:].decode(errors="replace"))
+                log("counter pass failed:", tag, r.returncode, r.stdout[-2000:].decode(errors="replace"))
                 return None
-            v = _pmc_kib(csvs[0], kern)
-            if v is None:
+            v = _pmc_means(csvs[0], kern)
+            if not all(c in v for c in counters):
+                log("counter pass without values:", tag, v)
                 return None
             res[tag] = v
     copy_kib = copy_mib * 1024.0
-    f_scale = copy_kib / res["copy_fetch"]
-    w_scale = copy_kib / res["copy_write"]
-    fetch_b = res["fetch"] * f_scale * 1024.0
-    write_b = res["write"] * w_scale * 1024.0
-    return {"bytes": fetch_b + write_b, "fetch_bytes": fetch_b, "write_bytes": write_b,
-            "fetch_kib_raw": res["fetch"], "write_kib_raw": res["write"],
-            "fetch_scale": f_scale, "write_scale": w_scale,
-            "calibration": f"device copy of {copy_mib} MiB reads FETCH_SIZE {res['copy_fetch']:.1f} KiB, "
-                           f"WRITE_SIZE {res['copy_write']:.1f} KiB"}
+    f_scale = copy_kib / res["copy_fetch"]["FETCH_SIZE"]
+    w_scale = copy_kib / res["copy_write"]["WRITE_SIZE"]
+    fetch_b = res["fetch"]["FETCH_SIZE"] * f_scale * 1024.0
+    write_b = res["write"]["WRITE_SIZE"] * w_scale * 1024.0
+    traffic = {"bytes": fetch_b + write_b, "fetch_bytes": fetch_b, "write_bytes": write_b,
+               "fetch_kib_raw": res["fetch"]["FETCH_SIZE"], "write_kib_raw": res["write"]["WRITE_SIZE"],
+               "fetch_scale": f_scale, "write_scale": w_scale,
+               "calibration": f"device copy of {copy_mib} MiB reads FETCH_SIZE {res['copy_fetch']['FETCH_SIZE']:.1f} "
+                              f"KiB, WRITE_SIZE {res['copy_write']['WRITE_SIZE']:.1f} KiB"}
+    issue = {"salu_insts": res["salu"]["SQ_INSTS_SALU"], "gpu_cycles": res["salu"]["GRBM_GUI_ACTIVE"]}
+    return {"traffic": traffic, "issue": issue}
 
 
 def serving_leg(args, pvvc) -> dict:
     """Serving workload: streams of mixed density (one third each 100k pts /
-    9-bit, 400k / 10-bit, 1M / 10-bit; GoP 8, qp 4), Poisson arrivals at 30 fps
-    per stream, continuous batching on the server, GoP state resident."""
+    9-bit, 400k / 10-bit, 1M / 10-bit; GoP 8, qp 4), each stream generated
+    from its own seed, Poisson arrivals at 30 fps per stream, continuous
+    batching on the server, GoP state resident."""
+    import concurrent.futures as cf
+
     from pvvc import serve
     n = args.serve_streams
     kinds = ["c3_100k", "c3_400k", "c3_1m"]
     counts = [n // 3, n // 3, n - 2 * (n // 3)]
+    plan = [(kind, 1000 + args.seed * 997 + 31 * i + j) for i, (kind, c) in enumerate(zip(kinds, counts))
+            for j in range(c)]
     t0 = time.time()
-    gops = {}
-    for i, kind in enumerate(kinds):
-        imgs = pvvc.synth_stream(kind, n_frames=8, seed=100 + i, threads=min(16, os.cpu_count() or 8))
-        gops[kind] = [pvvc.Frame.parse(b) for b in imgs]
-    log(f"serving: generated {len(kinds)} GoPs in {time.time() - t0:.1f}s")
-    specs = []
-    for kind, c in zip(kinds, counts):
-        specs += [serve.StreamSpec(kind, gops[kind]) for _ in range(c)]
+    workers = max(1, min(4, (os.cpu_count() or 8) // 4))
+
+    def gen(item):
+        kind, sd = item
+        imgs = pvvc.synth_stream(kind, n_frames=8, seed=sd, threads=4)
+        return kind, [pvvc.Frame.parse(b) for b in imgs]
+
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        streams = list(ex.map(gen, plan))
+    log(f"serving: generated {len(streams)} streams (one seed each) in {time.time() - t0:.1f}s")
+    specs = [serve.StreamSpec(kind, frames) for kind, frames in streams]
     srv = pvvc.Server(lanes=args.serve_lanes)
     st = serve.run(srv, specs, fps=30.0, duration=args.serve_seconds, seed=args.seed)
     srv.close()
-    st["pts_per_frame"] = {k: int(np.mean([f.points for f in g])) for k, g in gops.items()}
+    st["pts_per_frame"] = {k: int(np.mean([f.points for kk, fr in streams if kk == k for f in fr])) for k in kinds}
     st["streams_per_kind"] = dict(zip(kinds, counts))
+    st["seeds"] = f"{plan[0][1]}..{plan[-1][1]} (one per stream)"
     st["sustained"] = bool(st["frames_completed"] == st["frames_offered"] and st["frames_failed"] == 0
                            and st["p99_ms"] <= 33.0)
     for k in ("p50_ms", "p99_ms", "max_ms", "mpts_per_s", "offered_mpts_per_s", "submit_late_p99_ms",
@@ -136,6 +156,59 @@ def serving_leg(args, pvvc) -> dict:
         st[k] = round(float(st[k]), 3)
     log("serving", st)
     return st
+
+
+def c2_leg(args, pvvc) -> dict:
+    """The batched configuration: one 16-frame GoP of the 11-bit stream
+    (~3.3M points/frame, patches 50-20000) decoded by one persistent launch
+    per step, beside the per-frame drop-in decode of the same frames."""
+    gop = pvvc.SYNTH_PRESETS["c2"]["gop"]
+    t0 = time.time()
+    frames = [pvvc.Frame.parse(b) for b in pvvc.synth_stream("c2", n_frames=gop, seed=args.seed + 50,
+                                                              threads=min(16, os.cpu_count() or 8))]
+    log(f"c2: generated {gop} frames in {time.time() - t0:.1f}s")
+    metas = [f.meta() for f in frames]
+    n_patches = max(int(m["index"].max()) for m in metas) + 1
+    max_pts = max(int(m["size"].max()) for m in metas)
+    dec = pvvc.Decoder(n_patches, max_pts)
+    b = dec.batch(frames)
+    out = dec.outputs(b, pvvc.OUT_NATIVE)
+    dec.decode(b, pvvc.MODE_PERSISTENT, out)
+    pvvc.synchronize()
+    ok = bool((b.status() == 0).all())
+    wall, pk = [], []
+    for _ in range(args.c2_steps):
+        ts = time.perf_counter()
+        dec.decode(b, pvvc.MODE_PERSISTENT, out)
+        pvvc.synchronize()
+        wall.append((time.perf_counter() - ts) * 1e3)
+        pk.append(sum(pvvc.last_decode_kernel_times()))
+    state = pvvc.State(n_patches, max_pts)
+    staged = [pvvc.StagedFrame(f) for f in frames]
+    buf = pvvc.DeviceBuffer(24 * max(f.points for f in frames))
+    for sf in staged:  # warm
+        pvvc.launch_dropin(sf, state, buf)
+    pvvc.synchronize()
+    lat = []
+    for sf in staged:
+        ts = time.perf_counter()
+        pvvc.launch_dropin(sf, state, buf)
+        pvvc.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    gop_ms = float(np.mean(wall))
+    pf_fps = len(lat) / (sum(lat) * 1e-3)
+    res = {"frames": gop, "points_per_frame": int(np.mean([f.points for f in frames])),
+           "patches_per_frame": int(np.mean([len(m["size"]) for m in metas])),
+           "patch_points_min_max": [int(min(m["size"].min() for m in metas)), max_pts],
+           "status_ok": ok, "steps": args.c2_steps,
+           "batched_mpts_per_s": round(b.total_points / (gop_ms * 1e-3) / 1e6, 1),
+           "batched_frames_per_s": round(gop / (gop_ms * 1e-3), 1), "ms_per_gop": round(gop_ms, 3),
+           "persistent_kernel_ms": round(float(np.mean(pk)), 3),
+           "per_frame_dropin": {"frames_per_s": round(pf_fps, 1), "p50_ms": round(pct(lat, 50), 3),
+                                "p99_ms": round(pct(lat, 99), 3)},
+           "batched_vs_per_frame": round(gop / (gop_ms * 1e-3) / pf_fps, 3)}
+    log("c2", res)
+    return res
 
 
 def cpu_model() -> str:
@@ -154,7 +227,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--preset", default="c1")
-    ap.add_argument("--gops", type=int, default=1, help="distinct GoPs generated (cycled over steps)")
+    ap.add_argument("--gops", type=int, default=4,
+                    help="distinct GoPs (consecutive GoPs of one seeded stream), cycled over the steps")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -168,6 +242,8 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the multi-stream serving leg")
     ap.add_argument("--no-encode", action="store_true", help="skip the GPU colour-encoder leg")
     ap.add_argument("--no-quality", action="store_true", help="skip the MSE / PSNR leg")
+    ap.add_argument("--no-c2", action="store_true", help="skip the c2 (batched, 11-bit, GoP 16) leg")
+    ap.add_argument("--c2-steps", type=int, default=5)
     ap.add_argument("--serve-streams", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-lanes", type=int, default=3)
@@ -181,10 +257,11 @@ def main() -> int:
               "(patches shard by stream / patch-index range with no exchange).", file=sys.stderr)
         return 2
 
-    traffic = None
+    counters = None
     if not args.no_traffic and not args.calibrate_copy and args.only == "all":
-        traffic = measure_traffic(args)
-        log("traffic", traffic)
+        counters = measure_counters(args)
+        log("counters", counters)
+    traffic = counters["traffic"] if counters else None
 
     import pvvc
 
@@ -268,6 +345,24 @@ def main() -> int:
     cu = int(info["cu"])
     tot_sym = sum(sym_per_step[i % args.gops] for i in range(args.steps))
     sym_per_s_cu = tot_sym / (float(np.sum(pk_ms)) * 1e-3) / cu if pk_ms else float("nan")
+    # decode-issue bound: the RLGR loop runs on the scalar unit, one scalar
+    # instruction per cycle per CU at most; SQ_INSTS_SALU per launch over that
+    # capacity at the peak engine clock is the launch's floor
+    issue = None
+    if counters:
+        salu = counters["issue"]["salu_insts"]
+        cyc = counters["issue"]["gpu_cycles"]
+        sym_launch = float(np.mean(sym_per_step))
+        peak_rate = cu * ENGINE_CLOCK_HZ
+        issue = {"bound": "salu", "unit": "G scalar instructions/s",
+                 "salu_insts_per_launch": int(salu),
+                 "salu_per_symbol": round(salu / sym_launch, 3),
+                 "achieved": round(salu / (pk_avg_ms * 1e-3) / 1e9, 2),
+                 "peak": round(peak_rate / 1e9, 2),
+                 "frac": round(salu / (pk_avg_ms * 1e-3) / peak_rate, 4),
+                 "busy_frac_profiled": round(salu * XCDS / (cu * cyc), 4) if cyc else None,
+                 "floor_ms": round(salu / peak_rate * 1e3, 4),
+                 "cycles_per_xcd_profiled": int(cyc / XCDS)}
     log(f"persistent: {avg_kernel_ms:.3f} ms/GoP all launches, {pk_avg_ms:.3f} ms per persistent_kernel, "
         f"{np.mean(wall):.3f} ms wall, {batched_mpts:.1f} Mpts/s")
 
@@ -446,6 +541,11 @@ def main() -> int:
         quality = {"frames": per}
         log("quality", quality)
 
+    # ---------------- the batched configuration c2 ----------------
+    c2 = None
+    if run_b and not args.no_c2 and args.only == "all":
+        c2 = c2_leg(args, pvvc)
+
     # ---------------- serving loop: many concurrent streams ----------------
     serving = None
     if not args.no_serving and args.only == "all":
@@ -472,7 +572,7 @@ def main() -> int:
                 p.write_bytes(img)
                 paths.append(p)
             t_start = time.time()
-            while time.time() - t_start < args.cpu_seconds and nfr < 4 * len(frames):
+            while time.time() - t_start < args.cpu_seconds and nfr < 40 * len(frames):
                 k = nfr % len(frames)
                 of = binding.OracleFrame(str(paths[k]))
                 ms_ref, ms_all = od.decode_timing_only(of, threads, bufs)
@@ -526,7 +626,7 @@ def main() -> int:
         "data": "synthetic (seeded humanoid stream, generated on the host)",
         "config": {"workload": f"{args.preset}: synthetic humanoid, GoP {gop}, "
                                f"{int(np.mean(pts_per_frame))} pts/frame, {len(metas[0]['size'])} patches/frame",
-                   "frames_per_step": gop, "points_per_step": int(np.mean(step_pts)),
+                   "frames_per_step": gop, "points_per_step": int(np.mean(step_pts)), "gops": args.gops,
                    "patches_per_frame": int(np.mean([len(m['size']) for m in metas])),
                    "tuning": dec.tuning(), "autotuned": autotuned},
         "frames_per_s": round(batched_fps, 1),
@@ -540,8 +640,9 @@ def main() -> int:
                      "traffic": round(traffic["bytes"]) if traffic else None,
                      "kernel": "persistent_kernel", "alg_bytes_per_launch": int(alg_per_launch),
                      "kernel_ms": round(pk_avg_ms, 4), "launches": pk_launches,
-                     "traffic_detail": traffic},
+                     "traffic_detail": traffic, "issue": issue},
         "state_bytes_per_step": int(np.mean(state_bytes)),
+        "c2": c2,
         "rlgr_symbols_per_s_per_cu": round(sym_per_s_cu, 1),
         "end_to_end_per_frame": e2e,
         "encode": encode,
