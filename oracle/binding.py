@@ -57,6 +57,9 @@ def lib() -> C.CDLL:
     sig("orc_ctx_set_fp64", None, vp, C.c_int)
     sig("orc_decode_frame", C.c_int64, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp)
     sig("orc_cloud_mse", C.c_int, vp, vp, C.c_int64, vp, vp, C.c_int64, C.c_int, C.c_int, vp, vp, vp)
+    sig("orc_raht_forward", C.c_int, vp, C.c_uint32, vp, C.c_uint32, vp)
+    sig("orc_raht_inverse", C.c_int, vp, C.c_uint32, C.c_uint32, vp, vp)
+    sig("orc_color_encode", C.c_int64, vp, C.c_uint32, vp, C.c_uint32, C.c_uint8, vp, vp, C.c_int64)
     _lib = L
     return L
 
@@ -77,6 +80,39 @@ def rlgr_decode(data: bytes, nsym: int) -> tuple[np.ndarray, int]:
     out = np.zeros(max(1, nsym), np.int64)
     st = lib().orc_rlgr_decode(_buf(data), len(data), nsym, out.ctypes.data)
     return out[:nsym], st
+
+
+def raht_forward(geo: bytes, leaf_yuv) -> tuple[np.ndarray | None, int]:
+    """Forward RAHT (raht_octree.cc:189-247) of leaf colours [n, 3] on intra
+    geometry bytes: (coefficients [3, n] planar in stream order, status)."""
+    y = np.ascontiguousarray(leaf_yuv, np.float32).reshape(-1, 3)
+    n = len(y)
+    c = np.zeros(3 * max(1, n), np.float32)
+    st = lib().orc_raht_forward(_buf(geo), len(geo), y.ctypes.data, n, c.ctypes.data)
+    return (c[:3 * n].reshape(3, n) if st == 0 else None), int(st)
+
+
+def raht_inverse(geo: bytes, coef) -> tuple[np.ndarray | None, int]:
+    """Inverse RAHT of planar coefficients [3, n]: (leaf colours [n, 3], status)."""
+    c = np.ascontiguousarray(coef, np.float32).reshape(3, -1)
+    n = c.shape[1]
+    out = np.zeros((max(1, n), 3), np.float32)
+    st = lib().orc_raht_inverse(_buf(geo), len(geo), n, c.ctypes.data, out.ctypes.data)
+    return (out[:n] if st == 0 else None), int(st)
+
+
+def color_encode(geo: bytes, leaf_yuv, qp: int) -> tuple[bytes | None, np.ndarray | None, int]:
+    """Colour stream of one slice (GoP_encoding.cc:172-192, before zstd):
+    (RLGR bytes, quantised symbols [3n] planar, status)."""
+    y = np.ascontiguousarray(leaf_yuv, np.float32).reshape(-1, 3)
+    n = len(y)
+    q = np.zeros(3 * max(1, n), np.int64)
+    r = lib().orc_color_encode(_buf(geo), len(geo), y.ctypes.data, n, qp, q.ctypes.data, None, 0)
+    if r < 0:
+        return None, None, int(-r)
+    out = np.zeros(max(1, r), np.uint8)
+    lib().orc_color_encode(_buf(geo), len(geo), y.ctypes.data, n, qp, None, out.ctypes.data, r)
+    return out[:r].tobytes(), q[:3 * n], 0
 
 
 def tree_expand(geo: bytes, size: int) -> tuple[np.ndarray | None, int, int]:

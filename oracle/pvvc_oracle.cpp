@@ -10,6 +10,9 @@
  * Build: oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math) so that the
  * FP32 operation order is the reference's: x86-64 SSE scalar float, no FMA.
  *
+ * The forward path (raht_forward, quantise, orc_color_encode) restates the
+ * colour half of the encoder, the checker of the GPU colour encoder.
+ *
  * Pinning (see tests/test_oracle.py): RLGR by hand-derived known-answer
  * vectors and encode->decode round trips; the octree by closed-form leaf
  * centres (header (1,1,1)/(2,2,2)/h=2, occupancy FF and 81, h=1) and by a
@@ -910,6 +913,98 @@ void mse_sums(const float* qx, const uint8_t* qc, int64_t nq, const uint8_t* tc,
 /* ======================================================================== */
 /* C ABI for the Python tests (ctypes).                                      */
 /* ======================================================================== */
+/* ------------------------------------------------------------------------ */
+/* Forward path (the colour half of the encoder)                             */
+/* ------------------------------------------------------------------------ */
+
+/* HaarTransform: octree_base.cc:164-184.  FP32; T as the reference builds it,
+ * G = g0*T00 + g1*T01, H = g0*T10 + g1*T11 per channel (ColorYUV operator*
+ * then operator+, common.h:140-152). */
+void haar_fwd(int w0, int w1, const YUV& g0, const YUV& g1, YUV& G, YUV& H) {
+    if (w0 == 0 && w1 == 0) {
+        G = YUV{};
+        H = YUV{};
+        return;
+    }
+    float base = std::sqrt(static_cast<float>(w0 + w1));
+    float t00 = std::sqrt(static_cast<float>(w0)) / base;
+    float t01 = std::sqrt(static_cast<float>(w1)) / base;
+    float t10 = -std::sqrt(static_cast<float>(w1)) / base;
+    float t11 = std::sqrt(static_cast<float>(w0)) / base;
+    G.y = g0.y * t00 + g1.y * t01;
+    G.u = g0.u * t00 + g1.u * t01;
+    G.v = g0.v * t00 + g1.v * t01;
+    H.y = g0.y * t10 + g1.y * t11;
+    H.u = g0.u * t10 + g1.u * t11;
+    H.v = g0.v * t10 + g1.v * t11;
+}
+
+/* HierarchicalTransform: octree_base.cc:65-82.  Pairs i = 7..1 in NodeWeight
+ * order; raht[i] takes the low-pass, H[i] the high-pass; then raht[0] is the
+ * node's DC and raht[1..7] its high-passes. */
+void hier_fwd(Node& nd) {
+    YUV H[8];
+    for (int i = 7; i > 0; --i) {
+        int a = NodeWeight[i][0], b = NodeWeight[i][1];
+        YUV G, Hi;
+        haar_fwd(nd.weight[a], nd.weight[b], nd.raht[a], nd.raht[b], G, Hi);
+        nd.raht[i] = G;
+        H[i] = Hi;
+    }
+    nd.raht[0] = nd.raht[1];
+    for (int i = 1; i <= 7; ++i) nd.raht[i] = H[i];
+}
+
+/* RAHTOctree::RAHT: raht_octree.cc:189-247, on the tree of intra geometry
+ * bytes.  The decoder's MakeTree (invert_raht_octree.cc:122-201) rebuilds the
+ * writer's tree from those bytes exactly: the same layers, node order, child
+ * indices and weights (AddNode, raht_octree.cc:130-187, appends each layer in
+ * the order the decoder's breadth-first expansion reads it).  leaf[k] is leaf
+ * k's colour: the writer's average of the points in that voxel (:193-200),
+ * done by the caller.  Output: the coefficients in stream order (after the
+ * std::reverse of :245), planar Y[n] U[n] V[n].  Returns an ORC status. */
+int raht_forward(const uint8_t* geo, uint32_t geo_size, const float* leaf, uint32_t n, std::vector<YUV>& out) {
+    PatchDecoder d;
+    d.fp64 = false;
+    d.size = n;
+    std::vector<uint8_t> g(geo, geo + geo_size);
+    int st = d.make_tree(g);
+    if (st != ORC_OK) return st;
+    auto& tree = d.tree;
+    const int h = d.height;
+    for (size_t k = 0; k < tree.back().size(); ++k) /* :193-200 */
+        tree.back()[k].raht[0] = YUV{leaf[3 * k], leaf[3 * k + 1], leaf[3 * k + 2]};
+    for (int i = h - 2; i >= 0; --i) { /* :202-213 */
+        for (auto& node : tree[i]) {
+            for (int idx = 0; idx < 8; ++idx)
+                if (node.index[idx] != -1) node.raht[idx + 8] = tree[i + 1][node.index[idx]].raht[0];
+            hier_fwd(node);
+        }
+    }
+    std::vector<YUV> res; /* :215-245 */
+    res.push_back(tree.front().front().raht[0]);
+    for (int i = 0; i < h - 1; ++i)
+        for (auto& node : tree[i])
+            for (int idx = 1; idx < 8; ++idx)
+                if (node.weight[NodeWeight[idx][0]] != 0 && node.weight[NodeWeight[idx][1]] != 0)
+                    res.push_back(node.raht[idx]);
+    std::reverse(res.begin(), res.end());
+    if (res.size() != n) return ORC_ERR_UNMATCHED_SIZE;
+    out.swap(res);
+    return ORC_OK;
+}
+
+/* Quantisation: GoP_encoding.cc:180-186, std::round(c / QP) as int64
+ * (FIX_DATA_INT, entropy_codec.h:33); planar Y|U|V. */
+void quantise(const std::vector<YUV>& c, uint8_t qp, int64_t* q) {
+    const size_t n = c.size();
+    for (size_t j = 0; j < n; ++j) {
+        q[j] = static_cast<int64_t>(std::round(c[j].y / qp));
+        q[j + n] = static_cast<int64_t>(std::round(c[j].u / qp));
+        q[j + 2 * n] = static_cast<int64_t>(std::round(c[j].v / qp));
+    }
+}
+
 extern "C" {
 
 /* RLGR encode of n symbols; returns the byte count (writes at most cap). */
@@ -959,6 +1054,58 @@ int64_t orc_patch_get(void* h, float* xyz, float* yuv, uint8_t* rgb, float* rgbf
         if (rgb64 && d->fp64) convert_rgb64(d->colors64[i], rgb64 + 3 * i);
     }
     return static_cast<int64_t>(n);
+}
+
+/* Forward RAHT of leaf colours (n x 3, leaf order) on intra geometry bytes:
+ * 3n FP32 coefficients in stream order, planar.  Returns 0 or a status. */
+int orc_raht_forward(const uint8_t* geo, uint32_t geo_size, const float* leaf_yuv, uint32_t n, float* coef) {
+    std::vector<YUV> c;
+    int st = raht_forward(geo, geo_size, leaf_yuv, n, c);
+    if (st != ORC_OK) return st;
+    for (uint32_t j = 0; j < n; ++j) {
+        coef[j] = c[j].y;
+        coef[j + n] = c[j].u;
+        coef[j + 2 * n] = c[j].v;
+    }
+    return ORC_OK;
+}
+
+/* Inverse RAHT of 3n FP32 coefficients (stream order, planar) on intra
+ * geometry bytes: the decoder's InvertRAHT (invert_raht_octree.cc:256-317)
+ * without the RLGR and dequantisation steps.  leaf_yuv receives n x 3.
+ * Returns 0 or a status. */
+int orc_raht_inverse(const uint8_t* geo, uint32_t geo_size, uint32_t n, const float* coef, float* leaf_yuv) {
+    PatchDecoder d;
+    d.fp64 = false;
+    d.size = n;
+    std::vector<uint8_t> g(geo, geo + geo_size);
+    int st = d.make_tree(g);
+    if (st != ORC_OK) return st;
+    d.coefs.assign(n, YUV{});
+    for (uint32_t j = 0; j < n; ++j) d.coefs[j] = YUV{coef[j], coef[j + n], coef[j + 2 * n]};
+    d.invert_raht(false);
+    for (uint32_t k = 0; k < n; ++k) {
+        leaf_yuv[3 * k] = d.colors[k].y;
+        leaf_yuv[3 * k + 1] = d.colors[k].u;
+        leaf_yuv[3 * k + 2] = d.colors[k].v;
+    }
+    return ORC_OK;
+}
+
+/* The colour stream of one slice as GoP_encoding.cc:172-192 writes it before
+ * the optional zstd wrap: forward RAHT, quantisation, RLGR.  quant (optional)
+ * receives the 3n quantised symbols.  Returns the byte count (writes at most
+ * cap bytes to out), or -(status). */
+int64_t orc_color_encode(const uint8_t* geo, uint32_t geo_size, const float* leaf_yuv, uint32_t n, uint8_t qp,
+                         int64_t* quant, uint8_t* out, int64_t cap) {
+    if (qp == 0 || n == 0) return -ORC_ERR_BAD_SLICE;
+    std::vector<YUV> c;
+    int st = raht_forward(geo, geo_size, leaf_yuv, n, c);
+    if (st != ORC_OK) return -st;
+    std::vector<int64_t> q(3ull * n);
+    quantise(c, qp, q.data());
+    if (quant) std::copy(q.begin(), q.end(), quant);
+    return orc_rlgr_encode(q.data(), static_cast<int64_t>(q.size()), out, cap);
 }
 
 /* Tree-only expansion of intra geometry bytes (for known-answer tests).

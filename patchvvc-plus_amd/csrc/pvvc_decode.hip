@@ -1330,6 +1330,12 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
         slot->status = st;
     }
     PVVC_STAMP(item, 5);
+#ifdef PVVC_PHASE_PROFILE
+    /* where the item ran: HW_ID (wave, SIMD, CU, SH, SE) and the XCC id */
+    if (g_prof && threadIdx.x == 0)
+        g_prof[8ull * item + 6] = static_cast<uint64_t>(__builtin_amdgcn_s_getreg((31 << 11) | 4)) |
+                                  (static_cast<uint64_t>(__builtin_amdgcn_s_getreg((31 << 11) | 20)) << 32);
+#endif
     return st;
 }
 

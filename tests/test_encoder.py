@@ -1,6 +1,13 @@
 """Forward path: the GPU colour encoder (forward RAHT, quantisation, RLGR
-writer) against the stream writer's CPU encoder, byte for byte, on
-surface-like voxel patches with intra-like and residual-like colours."""
+writer) against the oracle's restatement of the encoder's colour half
+(oracle/pvvc_oracle.cpp orc_color_encode: raht_octree.cc:189-247,
+GoP_encoding.cc:172-192, entropy_codec.cc:65-159), byte for byte, on
+surface-like voxel patches with intra-like and residual-like colours and on
+every colour stream of a decoded c1 frame.  The stream writer's CPU encoder
+(product code, the synthetic-stream generator) only builds the inputs here,
+and is itself checked against the oracle."""
+import sys
+
 import numpy as np
 import pytest
 
@@ -22,7 +29,9 @@ def _surface_patch(rng, n_target):
     return np.array(sorted(pts), np.float32)
 
 
-def _jobs(pvvc, seed=3, count=40):
+def _jobs(pvvc, oracle, seed=3, count=40):
+    """(octree bytes, leaf YUV, qp) jobs and the oracle's bytes for each; the
+    writer's own bytes must equal them too."""
     rng = np.random.default_rng(seed)
     jobs, refs = [], []
     sizes = [1, 2, 3, 9] + list(rng.integers(10, 3000, count - 4))
@@ -39,15 +48,17 @@ def _jobs(pvvc, seed=3, count=40):
         # leaves may hold several points (the tree's range is max - min, so
         # neighbouring voxels can share a leaf); the encoders take leaf colours
         assert 1 <= len(leaf) <= m
+        ref, _, st = oracle.color_encode(geo, leaf, qp)
+        assert st == 0 and ref == col
         jobs.append((geo, leaf, qp))
-        refs.append(col)
+        refs.append(ref)
     return jobs, refs
 
 
 def test_cpu_encoder_decodes_with_oracle(pvvc, oracle):
     """The CPU reference encoder's stream decodes (oracle) to the RAHT of the
     leaf colours within the quantisation step."""
-    jobs, refs = _jobs(pvvc, seed=5, count=8)
+    jobs, refs = _jobs(pvvc, oracle, seed=5, count=8)
     for (geo, leaf, qp), col in zip(jobs, refs):
         p = oracle.Patch()
         assert p.set_slice(0x01, len(leaf), qp, geo, col) == 0
@@ -58,8 +69,8 @@ def test_cpu_encoder_decodes_with_oracle(pvvc, oracle):
 
 
 @pytest.mark.gpu
-def test_gpu_encoder_matches_cpu(pvvc):
-    jobs, refs = _jobs(pvvc)
+def test_gpu_encoder_matches_oracle(pvvc, oracle):
+    jobs, refs = _jobs(pvvc, oracle)
     outs, status, ms = pvvc.encode_colors(jobs)
     assert status == [0] * len(jobs), status
     for i, (o, r) in enumerate(zip(outs, refs)):
@@ -68,23 +79,23 @@ def test_gpu_encoder_matches_cpu(pvvc):
 
 
 @pytest.mark.gpu
-def test_gpu_encoder_rejects_bad_geometry(pvvc):
-    geo, leaf, qp = _jobs(pvvc, seed=9, count=5)[0][4]
+def test_gpu_encoder_rejects_bad_geometry(pvvc, oracle):
+    geo, leaf, qp = _jobs(pvvc, oracle, seed=9, count=5)[0][4]
     bad = bytearray(geo)
     bad[24] = 0  # height 0
     outs, status, _ = pvvc.encode_colors([(bytes(bad), leaf, qp), (geo, leaf[:-1], qp)])
     assert status == [1, 3]  # BAD_SLICE, UNMATCHED_SIZE (leaf count != n)
 
 
-def test_cpu_encoder_on_tree_equals_point_encoder(pvvc):
-    """The CPU encoder on given octree bytes (the reference for GPU parity at
-    full size) produces the same stream as the writer's encoder from points."""
-    jobs, refs = _jobs(pvvc, seed=11, count=12)
+def test_cpu_encoder_on_tree_equals_point_encoder(pvvc, oracle):
+    """The writer's encoder on given octree bytes produces the same stream as
+    the writer's encoder from points (and as the oracle)."""
+    jobs, refs = _jobs(pvvc, oracle, seed=11, count=12)
     for (geo, leaf, qp), col in zip(jobs, refs):
         assert pvvc.encode_colors_cpu_on_tree(geo, leaf, qp) == col
 
 
-def _boundary_jobs(pvvc):
+def _boundary_jobs(pvvc, oracle):
     """Streams whose byte length ends on or just short of a 256-byte (64-word)
     boundary: the writer's last partial word then completes or nearly
     completes a store chunk.  Found by a seeded search over small random
@@ -99,21 +110,23 @@ def _boundary_jobs(pvvc):
         geo, leaf, col = pvvc.encode_colors_cpu(v, yuv, qp)
         r = len(col) % 256
         if len(col) >= 256 and r in want and want[r] is None:
-            want[r] = ((geo, leaf, qp), col)
+            ref, _, st = oracle.color_encode(geo, leaf, qp)
+            assert st == 0 and ref == col
+            want[r] = ((geo, leaf, qp), ref)
         if all(x is not None for x in want.values()):
             break
     assert all(x is not None for x in want.values()), {k: x is not None for k, x in want.items()}
     return [x[0] for x in want.values()], [x[1] for x in want.values()]
 
 
-def test_boundary_search_finds_every_residue(pvvc):
-    jobs, refs = _boundary_jobs(pvvc)
+def test_boundary_search_finds_every_residue(pvvc, oracle):
+    jobs, refs = _boundary_jobs(pvvc, oracle)
     assert sorted(len(r) % 256 for r in refs) == [0, 1, 252, 253, 254, 255]
 
 
 @pytest.mark.gpu
-def test_gpu_encoder_chunk_boundaries(pvvc):
-    jobs, refs = _boundary_jobs(pvvc)
+def test_gpu_encoder_chunk_boundaries(pvvc, oracle):
+    jobs, refs = _boundary_jobs(pvvc, oracle)
     outs, status, _ = pvvc.encode_colors(jobs)
     assert status == [0] * len(jobs), status
     for o, r in zip(outs, refs):
@@ -133,16 +146,16 @@ def test_packed_encoder_rejects_inconsistent_jobs(pvvc):
 
 
 @pytest.mark.gpu
-def test_gpu_encoder_kept_buffers(pvvc):
+def test_gpu_encoder_kept_buffers(pvvc, oracle):
     """Calls in a row reuse the library's device and pinned buffers: a large
     call (~1.2 M leaves, ~14 MB of input, several staging chunks), a smaller
     one, the large one again in reverse order, then again after the buffers
-    are released.  Every stream equals the CPU writer's bytes."""
-    jobs, refs = _jobs(pvvc, seed=31, count=40)
+    are released.  Every stream equals the oracle's bytes."""
+    jobs, refs = _jobs(pvvc, oracle, seed=31, count=40)
     reps = 20
     big, big_refs = jobs * reps, refs * reps
     assert sum(len(j[1]) for j in big) * 12 > (8 << 20)
-    small, small_refs = _boundary_jobs(pvvc)
+    small, small_refs = _boundary_jobs(pvvc, oracle)
     for name, js, rs in [("big", big, big_refs), ("small", small, small_refs),
                          ("big reversed", big[::-1], big_refs[::-1]), ("small", small, small_refs)]:
         outs, status, _ = pvvc.encode_colors(js)
@@ -155,8 +168,8 @@ def test_gpu_encoder_kept_buffers(pvvc):
 
 
 @pytest.mark.gpu
-def test_gpu_packed_encoder_equals_job_list(pvvc):
-    jobs, refs = _jobs(pvvc, seed=41, count=24)
+def test_gpu_packed_encoder_equals_job_list(pvvc, oracle):
+    jobs, refs = _jobs(pvvc, oracle, seed=41, count=24)
     geo_all = np.frombuffer(b"".join(bytes(j[0]) for j in jobs), np.uint8)
     gsz = [len(j[0]) for j in jobs]
     yuv_all = np.concatenate([j[1] for j in jobs])
@@ -165,3 +178,32 @@ def test_gpu_packed_encoder_equals_job_list(pvvc):
     out_all, off, size, status, ms = pvvc.encode_colors_packed(geo_all, gsz, yuv_all, npts, qp)
     assert (status == 0).all() and ms > 0
     assert [out_all[o:o + z].tobytes() for o, z in zip(off, size)] == refs
+
+
+@pytest.mark.gpu
+def test_gpu_encoder_c1_frame_streams_match_oracle(pvvc, oracle):
+    """Every colour stream of a c1 intra frame (~860 patches of 200-3000
+    points): the octree bytes of each slice, its leaf colours as the oracle
+    decodes them, and its qp, encoded on the GPU and by the oracle; plus a
+    signed residual-like colour set on each of the same trees."""
+    imgs = pvvc.synth_stream("c1", n_frames=1, seed=17, threads=16)
+    f0 = pvvc.Frame.parse(imgs[0])
+    m0 = f0.meta()
+    jobs, want = [], []
+    for s in range(f0.slice_cnt):
+        geo = f0.geometry(s)
+        n, qp = int(m0["size"][s]), int(m0["qp"][s])
+        pt = oracle.Patch()
+        assert pt.set_slice(0x01, n, qp, geo, f0.color(s)) == 0
+        leaf = pt.get(n)["yuv"]
+        for yuv in (leaf, leaf - leaf[::-1]):
+            jobs.append((geo, yuv, qp))
+            ref, _, st = oracle.color_encode(geo, yuv, qp)
+            assert st == 0
+            want.append(ref)
+    print(f"c1 frame 0: {f0.slice_cnt} slices, {len(jobs)} colour streams", flush=True)
+    sys.stdout.flush()
+    outs, status, ms = pvvc.encode_colors(jobs)
+    assert status == [0] * len(jobs)
+    bad = [i for i, (o, r) in enumerate(zip(outs, want)) if o != r]
+    assert not bad, f"{len(bad)} of {len(jobs)} streams differ, first {bad[:5]}"

@@ -10,6 +10,10 @@ by checks that a plausible restatement mistake fails:
   from-scratch recursive expansion written here.
 * Inverse RAHT: the two-leaf closed form, the constant-colour identity
   (DC = sqrt(n) * c, AC = 0) and forward->inverse identity, against FP64.
+* Forward RAHT + quantisation (the GPU encoder's oracle): the two-leaf
+  closed form in exact FP32, the constant-colour identity, identity through
+  the pinned inverse with Parseval's equality, and round-half-away-from-zero
+  known answers on single-leaf patches.
 * ConvertRGB: BT.601 inverse matrix values and rounding at .5 boundaries.
 """
 import math
@@ -303,6 +307,108 @@ def test_iraht_forward_inverse_identity(oracle, pvvc):
         err = np.abs(r["yuv"][order] - yuv[pin])
         assert np.sqrt((err ** 2).mean()) < 0.6
         assert err.max() < 4.0
+
+
+# --------------------------------------------------------------------------
+# Forward RAHT and quantisation (the oracle of the GPU colour encoder)
+# --------------------------------------------------------------------------
+def test_forward_two_leaf_closed_form(oracle):
+    """Occupancy 81 (children 0 and 7 of one node): every pair but the root's
+    passes one side through (factors 0 and 1), and the root pair has weights
+    (1, 1), so the stream is [AC, DC] = [a*(-t) + b*t, a*t + b*t] with
+    t = sqrt(1)/sqrt(2) in FP32 (HaarTransform, octree_base.cc:164-184;
+    the final std::reverse, raht_octree.cc:245)."""
+    geo = header((1, 1, 1), (2, 2, 2), 2) + b"\x81"
+    a = np.array([10.25, -3.5, 200.0], np.float32)
+    b = np.array([7.0, 91.75, -0.5], np.float32)
+    c, st = oracle.raht_forward(geo, np.stack([a, b]))
+    assert st == 0
+    t = np.float32(np.sqrt(np.float32(1))) / np.sqrt(np.float32(2))
+    np.testing.assert_array_equal(c[:, 0], a * (-t) + b * t)
+    np.testing.assert_array_equal(c[:, 1], a * t + b * t)
+    np.testing.assert_allclose(c[:, 1], (a.astype(np.float64) + b) / math.sqrt(2), rtol=1e-6)
+
+
+def test_forward_constant_colour(oracle):
+    """A constant colour c over n leaves: DC = sqrt(n) * c, every AC 0."""
+    rng = np.random.default_rng(4)
+    for t in range(12):
+        geo, n = _random_geometry(rng, int(rng.integers(2, 8)), float(rng.uniform(0.2, 0.7)))
+        col = np.array([100.0, 120.0, 140.0], np.float32)
+        c, st = oracle.raht_forward(geo, np.tile(col, (n, 1)))
+        assert st == 0
+        np.testing.assert_allclose(c[:, -1], math.sqrt(n) * col, rtol=2e-6)
+        np.testing.assert_allclose(c[:, :-1], 0, atol=1e-3 * math.sqrt(n))
+
+
+def test_forward_inverse_identity(oracle):
+    """The oracle's inverse (pinned above by its own closed forms) undoes the
+    forward transform on random trees to FP32 accuracy: a wrong sign, a
+    transposed factor, a wrong weight pair or coefficient order fails."""
+    rng = np.random.default_rng(6)
+    for t in range(30):
+        geo, n = _random_geometry(rng, int(rng.integers(1, 9)), float(rng.uniform(0.1, 0.8)))
+        leaf = rng.uniform(-50, 300, (n, 3)).astype(np.float32)
+        c, st = oracle.raht_forward(geo, leaf)
+        assert st == 0
+        back, st = oracle.raht_inverse(geo, c)
+        assert st == 0
+        np.testing.assert_allclose(back, leaf, atol=2e-3 * max(1.0, math.log2(n + 1)))
+        # Parseval: the transform is orthonormal
+        np.testing.assert_allclose((c.astype(np.float64) ** 2).sum(), (leaf.astype(np.float64) ** 2).sum(),
+                                   rtol=1e-5)
+
+
+def test_forward_errors(oracle):
+    geo = header((1, 1, 1), (2, 2, 2), 2) + b"\x81"
+    assert oracle.raht_forward(geo, np.zeros((3, 3), np.float32))[1] == 3  # leaf count != n
+    assert oracle.raht_forward(geo[:20], np.zeros((2, 3), np.float32))[1] == 1  # short header
+    assert oracle.color_encode(geo, np.zeros((2, 3), np.float32), 0)[2] == 1  # qp 0
+
+
+def test_quantisation_rounds_half_away_from_zero(oracle):
+    """A single-leaf patch (height 1) codes its colour as the DC; with qp 4,
+    10/4 = 2.5 -> 3, -10/4 -> -3, 6/4 = 1.5 -> 2 (std::round on c / QP,
+    GoP_encoding.cc:180-186), and the bytes are the RLGR of those symbols."""
+    geo = header((0, 0, 0), (1, 1, 1), 1)
+    col, q, st = oracle.color_encode(geo, np.array([[10.0, -10.0, 6.0]], np.float32), 4)
+    assert st == 0
+    assert q.tolist() == [3, -3, 2]
+    assert col == oracle.rlgr_encode([3, -3, 2])
+    col, q, st = oracle.color_encode(geo, np.array([[9.99, -9.99, 5.99]], np.float32), 4)
+    assert q.tolist() == [2, -2, 1]
+
+
+def test_colour_stream_round_trip(oracle):
+    """Encode then decode (both oracle): leaf colours come back within the
+    quantisation bound of an orthonormal transform."""
+    rng = np.random.default_rng(9)
+    for t in range(10):
+        geo, n = _random_geometry(rng, int(rng.integers(2, 7)), float(rng.uniform(0.2, 0.7)))
+        leaf = rng.uniform(0, 255, (n, 3)).astype(np.float32)
+        qp = int(rng.choice([1, 2, 4, 8]))
+        col, q, st = oracle.color_encode(geo, leaf, qp)
+        assert st == 0
+        p = oracle.Patch()
+        assert p.set_slice(0x01, n, qp, geo, col) == 0
+        out = p.get(n)
+        err = np.abs(out["yuv"].astype(np.float64) - leaf)
+        assert err.max() <= qp * math.sqrt(3 * n) / 2 + 1e-2
+
+
+def test_product_writer_matches_oracle_forward(oracle, pvvc):
+    """The product's CPU stream writer (the synthetic-stream generator) writes
+    the oracle's colour bytes on the same tree and leaf colours."""
+    rng = np.random.default_rng(12)
+    for t in range(25):
+        geo, n = _random_geometry(rng, int(rng.integers(1, 8)), float(rng.uniform(0.1, 0.8)))
+        leaf = (128 + rng.normal(0, rng.uniform(1, 60), (n, 3))).astype(np.float32)
+        if t % 3 == 2:
+            leaf = rng.normal(0, 8, (n, 3)).astype(np.float32)
+        qp = int(rng.choice([1, 2, 4, 8, 16]))
+        col, _, st = oracle.color_encode(geo, leaf, qp)
+        assert st == 0
+        assert pvvc.encode_colors_cpu_on_tree(geo, leaf, qp) == col, (t, n, qp)
 
 
 # --------------------------------------------------------------------------
