@@ -349,3 +349,115 @@ def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> st
             return None
         if used > consumed:
             return f"loop consumed {consumed} bits, not at an event boundary (decoder at {used})"
+
+
+# ----------------------------------------------------------------------
+# Static wait-state check of the loop text
+# ----------------------------------------------------------------------
+# Rules the loop relies on (AMD Instinct MI300 / CDNA3 ISA, "manually
+# inserted wait states"; gfx950 keeps them):
+#   * a VALU instruction that writes an SGPR (v_readlane) followed by
+#     v_readlane / v_writelane using that SGPR as the lane select: 4 wait
+#     states;
+#   * the loop applies the same 4 wait states when the lane select was written
+#     by a scalar instruction (m0 for v_writelane, s88 for v_readlane);
+#   * a VALU-written SGPR read by a vector memory instruction: 5 (the loop has
+#     no memory instructions, which the check also asserts).
+# A scalar instruction reading an SGPR that v_readlane wrote is interlocked by
+# the hardware and needs none.  One instruction is one wait state; s_nop N is
+# N + 1.
+LANE_SELECT_WAIT = 4
+
+
+def _regs(x: str) -> set[str]:
+    m = re.fullmatch(r"s\[(\d+):(\d+)\]", x)
+    if m:
+        return {f"s{i}" for i in range(int(m.group(1)), int(m.group(2)) + 1)}
+    if re.fullmatch(r"s\d+", x) or x == "m0" or x.startswith("%["):
+        return {x}
+    return set()
+
+
+def _writes(op: str, a: list[str]) -> tuple[set[str], str]:
+    """SGPRs an instruction writes and the writer kind ("salu" / "valu")."""
+    if op == "v_readlane_b32":
+        return _regs(a[0]), "valu"
+    if op.startswith("s_") and a and not op.startswith(("s_cmp", "s_bitcmp", "s_cbranch", "s_branch", "s_nop")):
+        return _regs(a[0]), "salu"
+    return set(), ""
+
+
+def _lane_select(op: str, a: list[str]) -> set[str]:
+    return _regs(a[2]) if op in ("v_readlane_b32", "v_writelane_b32") else set()
+
+
+def _wait_states(op: str, a: list[str]) -> int:
+    return int(a[0], 0) + 1 if op == "s_nop" else 1
+
+
+def _predecessors(loop: "Loop") -> list[list[int]]:
+    pred = [[] for _ in loop.ins]
+    for i, (op, a) in enumerate(loop.ins):
+        if i + 1 < len(loop.ins) and op != "s_branch":
+            pred[i + 1].append(i)
+        if op in ("s_branch", "s_cbranch_scc0", "s_cbranch_scc1"):
+            t = loop.target(i, a[0])
+            if t < len(loop.ins):
+                pred[t].append(i)
+    return pred
+
+
+def hazard_report(lines: list[str]) -> list[str]:
+    """Every lane-select use whose last writer, on some path through the
+    loop's control flow, lies fewer than LANE_SELECT_WAIT wait states before
+    it; plus any memory instruction.  Empty when the loop keeps the rules."""
+    loop = Loop(lines)
+    pred = _predecessors(loop)
+    bad = []
+    for u, (op, a) in enumerate(loop.ins):
+        if op.startswith(("buffer_", "global_", "flat_", "s_load", "s_store", "s_buffer", "ds_", "s_atomic")):
+            bad.append(f"{u}: memory instruction {op}")
+        for r in _lane_select(op, a):
+            # backward walk: (index of the instruction before the use, wait states so far)
+            stack, seen = [(p, 0) for p in pred[u]], set()
+            if not pred[u]:
+                bad.append(f"{u}: {op} lane select {r} reaches the loop entry unwritten")
+            while stack:
+                i, d = stack.pop()
+                if (i, d) in seen:
+                    continue
+                seen.add((i, d))
+                w, kind = _writes(*loop.ins[i])
+                if r in w:
+                    if d < LANE_SELECT_WAIT:
+                        bad.append(f"{u}: {op} reads lane select {r} {d} wait states after the {kind} write at {i}")
+                    continue
+                d2 = d + _wait_states(*loop.ins[i])
+                if d2 >= LANE_SELECT_WAIT:
+                    continue
+                if not pred[i]:
+                    bad.append(f"{u}: {op} lane select {r} may come from before the loop ({d2} wait states)")
+                stack.extend((p, d2) for p in pred[i])
+    return bad
+
+
+def lane_select_distances(lines: list[str]) -> dict[int, int]:
+    """Least wait states, over all paths, between each lane-select use and its
+    writer (for the record in the test output)."""
+    loop = Loop(lines)
+    pred = _predecessors(loop)
+    out = {}
+    for u, (op, a) in enumerate(loop.ins):
+        for r in _lane_select(op, a):
+            best, stack, seen = 1 << 30, [(p, 0) for p in pred[u]], set()
+            while stack:
+                i, d = stack.pop()
+                if (i, d) in seen or d >= best:
+                    continue
+                seen.add((i, d))
+                if r in _writes(*loop.ins[i])[0]:
+                    best = min(best, d)
+                    continue
+                stack.extend((p, d + _wait_states(*loop.ins[i])) for p in pred[i])
+            out[u] = best
+    return out

@@ -89,3 +89,31 @@ def test_planted_loop_mistakes_are_caught(oracle, old, new):
     lines = lines[:i] + [new] + lines[i + 1:]
     fails, _, _ = _check(oracle, lines, streams_per_kind=1, states_per_stream=20)
     assert fails, f"planted '{new}' went unnoticed"
+
+
+def test_loop_wait_states():
+    """Every v_readlane / v_writelane lane select in the loop is written at
+    least 4 wait states earlier on every control-flow path, and the loop holds
+    no memory instruction (static check of the source text)."""
+    lines = R.extract_loop()
+    assert R.hazard_report(lines) == []
+    d = R.lane_select_distances(lines)
+    assert d and min(d.values()) >= R.LANE_SELECT_WAIT
+    print("least wait states before each lane-select use:", sorted(set(d.values())))
+
+
+def test_wait_state_check_catches_planted_violations():
+    """The checker fails loops whose lane select is written too close to its
+    use: s88 recomputed two instructions before a refill's v_readlane, or an
+    m0 update right before a v_writelane."""
+    lines = R.extract_loop()
+    i = lines.index("s_nop 1")
+    close = lines[:i - 1] + ["s_and_b32 s88, %[wpos], 63"] + lines[i - 1:]
+    assert any("lane select s88" in b for b in R.hazard_report(close))
+    # the s_nop alone is slack: without it the refill keeps 4 wait states
+    cut = lines[:i] + lines[i + 1:]
+    assert R.hazard_report(cut) == []
+    # an m0 write right before a v_writelane
+    j = lines.index("v_writelane_b32 %[acc], s89, m0")
+    moved = lines[:j] + ["s_add_u32 m0, m0, 0"] + lines[j:]
+    assert any("lane select m0" in b for b in R.hazard_report(moved))
