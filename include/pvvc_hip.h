@@ -199,7 +199,7 @@ void pvvc_host_free(void* ptr);
 typedef struct pvvc_decoder pvvc_decoder;
 typedef struct pvvc_batch pvvc_batch;
 
-/* Tunables (autotuner output).  Zero fields take defaults. */
+/* Tunables (autotuner output).  pvvc_decoder_set_tuning keeps the current value of a zero field. */
 typedef struct pvvc_tuning {
     int waves_per_patch;   /* 1, 2 or 4 waves per work item */
     int lds_bytes;         /* LDS workspace per work item (4..64 KiB) */
@@ -209,6 +209,11 @@ typedef struct pvvc_tuning {
                               wave priority (s_setprio 3), the rest of the queue in thirds at
                               2, 1 and 0; -1 = all at 0; default: N = one item per persistent
                               workgroup (the items that start together) */
+    int lane_max_points;   /* persistent mode: slices of at most this many points decode their
+                              RLGR one slice per lane (vector unit) on a second stream of the
+                              decoder, beside the scalar decoder of the longer slices, and join
+                              the caller's stream before the call returns; 0 = off (default),
+                              and -1 switches it off in pvvc_decoder_set_tuning */
 } pvvc_tuning;
 
 int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_patch);

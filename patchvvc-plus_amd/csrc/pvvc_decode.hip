@@ -5,6 +5,7 @@
  */
 #include "pvvc_internal.h"
 #include "pvvc_kernels.h"
+#include "pvvc_lane.h"
 #include "pvvc_runtime.h"
 
 #include <hip/hip_runtime.h>
@@ -1113,9 +1114,13 @@ struct Arena {
     }
 };
 
+/* pre_coef (optional): coefficients of the launch's items already decoded by
+ * lane_rlgr_kernel, at 3 * stage_off (planar, stream order), with their RLGR
+ * statuses in pre_status[item]; the slice then skips its own RLGR. */
 template <int WAVES>
 __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar, const Sink& sink, LevelMeta& meta,
-                            uint32_t launch_id, bool allow_wait, uint32_t item) {
+                            uint32_t launch_id, bool allow_wait, uint32_t item, const float* pre_coef = nullptr,
+                            const int* pre_status = nullptr) {
     PVVC_STAMP(item, 0);
     constexpr uint32_t NT = 64u * WAVES;
     const uint32_t tid = threadIdx.x;
@@ -1159,7 +1164,7 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
          * loads at the end of that chain. */
         fc = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
         wt = static_cast<uint32_t*>(ar.take(4ull * imax + 4));
-        coef = static_cast<float*>(ar.take(12ull * a.n));
+        coef = pre_coef ? const_cast<float*>(pre_coef) + 3ull * a.stage_off : static_cast<float*>(ar.take(12ull * a.n));
         if (!coef || !fc || !wt) st = PVVC_ERR_CAPACITY;
     }
 
@@ -1171,12 +1176,12 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
             if (ts == PVVC_OK) tree_weights(a.geo, fc, wt, meta);
             PVVC_STAMP(item, 1);
             int rs = PVVC_OK;
-            if (ts == PVVC_OK) rs = rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
+            if (ts == PVVC_OK) rs = pre_coef ? pre_status[item] : rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
             PVVC_STAMP(item, 2);
             st = ts != PVVC_OK ? ts : rs;
         } else {
             if (wave == 0) {
-                int rs = rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
+                int rs = pre_coef ? pre_status[item] : rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
                 if (lane_id() == 0) meta.rlgr_status = rs;
             } else if (wave == 1) {
                 int ts = tree_structure(a.geo, a.geo_size, a.n, fc, meta);
@@ -1429,7 +1434,8 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
                                                               uint32_t* queue, SlotDev* slots, Sink sink,
                                                               uint32_t lds_bytes, uint8_t* slab, uint64_t slab_bytes,
                                                               uint32_t launch_id, int* __restrict__ status,
-                                                              float* stage, uint32_t chunk, uint32_t prio_items) {
+                                                              float* stage, uint32_t chunk, uint32_t prio_items,
+                                                              const float* pre_coef, const int* pre_status) {
     __shared__ LevelMeta meta;
     __shared__ uint32_t next;
     uint8_t* ws = slab + static_cast<uint64_t>(blockIdx.x) * slab_bytes;
@@ -1461,7 +1467,7 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
             if (lvl == 3) __builtin_amdgcn_s_setprio(3);
             else if (lvl == 2) __builtin_amdgcn_s_setprio(2);
             else if (lvl == 1) __builtin_amdgcn_s_setprio(1);
-            int st = decode_slice<WAVES>(a, slot, ar, sink, meta, launch_id, true, i);
+            int st = decode_slice<WAVES>(a, slot, ar, sink, meta, launch_id, true, i, pre_coef, pre_status);
             if (lvl) __builtin_amdgcn_s_setprio(0);
             if (threadIdx.x == 0) status[i] = st;
             __syncthreads();
@@ -1474,6 +1480,31 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
         atomicExch(queue, 0u);
         atomicExch(queue + 1, 0u);
     }
+}
+
+/* RLGR of short slices, one lane per slice (rlgr_decode_lane); the rest of
+ * their decode runs in persistent_kernel with these coefficients.  Wavefront
+ * w takes list[64 w, 64 w + 64): it zeroes their coefficient ranges together
+ * (coalesced), then each lane decodes its own slice into coef_all at
+ * 3 * stage_off and leaves the status in lane_status[item]. */
+__global__ __launch_bounds__(64) void lane_rlgr_kernel(const ItemDev* __restrict__ items,
+                                                       const uint32_t* __restrict__ list, uint32_t count,
+                                                       float* __restrict__ coef_all, int* __restrict__ lane_status) {
+    const uint32_t base = blockIdx.x * 64u;
+    if (base >= count) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cnt = count - base < 64u ? count - base : 64u;
+    for (uint32_t j = 0; j < cnt; ++j) {
+        const ItemDev* it = items + list[base + j];
+        float* c = coef_all + 3ull * it->stage_off;
+        const uint32_t n3 = 3u * it->n;
+        for (uint32_t x = lane; x < n3; x += 64u) c[x] = 0.0f;
+    }
+    __threadfence_block(); /* the zeroes are written before any lane's coefficients */
+    if (lane >= cnt) return;
+    const uint32_t i = list[base + lane];
+    const ItemDev* it = items + i;
+    lane_status[i] = rlgr_decode_lane(it->col, it->col_size, 3u * it->n, it->qp, coef_all + 3ull * it->stage_off);
 }
 
 /* Zeroes a queue's counters once, after allocation. */
@@ -2264,7 +2295,7 @@ void fill_batch_image(const BatchPlan& p, const pvvc_frame* const* frames, uint8
 /* ---- native decoder / batch ---- */
 struct pvvc_decoder {
     StateInfo st;
-    pvvc_tuning tuning{1, 8 * 1024, 16, 1, 0};
+    pvvc_tuning tuning{1, 8 * 1024, 16, 1, 0, 0};
     bool auto_small = true;      /* small segments pick their own shape until a tuning is set */
     uint32_t launch_id = 0;
     uint8_t* slab = nullptr;
@@ -2277,6 +2308,17 @@ struct pvvc_decoder {
     bool has_stream = false;     /* pvvc_decoder_set_stream; else the process default */
     hipStream_t stream = nullptr;
     int white_small = -1;        /* pvvc_decoder_set_white_small_patches; -1 = process default */
+    /* lane decoding of short slices (tuning.lane_max_points): a second stream
+     * with its own queue and workspaces, and the lanes' coefficients */
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    uint32_t* queue2 = nullptr;
+    uint8_t* slab2 = nullptr;
+    uint64_t slab2_total = 0;
+    float* lane_coef = nullptr;
+    uint64_t lane_coef_bytes = 0;
+    int* lane_status = nullptr;
+    uint64_t lane_status_bytes = 0;
 };
 
 namespace {
@@ -2324,15 +2366,15 @@ void launch_frame(const pvvc_batch* b, int f, const Sink& sink, uint32_t lds, hi
 template <int W>
 void launch_persistent_w(const BatchDev& b, const Segment& sg, const Sink& sink, uint32_t lds, uint32_t grid,
                          uint32_t chunk, uint32_t prio, const LaunchRes& r, uint32_t launch_id, hipStream_t s) {
-    /* untimed launches (grid_cap set: the scratch-backed entry point) record no events */
-    const bool timed = r.grid_cap == 0;
+    const bool timed = r.timed;
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (timed) {
         ev = g_ktiming.next();
         hipEventRecord(ev.first, s);
     }
     hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b.items, b.order + sg.begin, sg.count,
-                       r.queue, r.slots, sink, lds, r.slab, r.slab_bytes, launch_id, b.status, r.stage, chunk, prio);
+                       r.queue, r.slots, sink, lds, r.slab, r.slab_bytes, launch_id, b.status, r.stage, chunk, prio,
+                       r.pre_coef, r.pre_status);
     if (timed) hipEventRecord(ev.second, s);
     if (sg.dcount)
         hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, b.items, b.defer + sg.dbegin, sg.dcount,
@@ -2382,13 +2424,13 @@ bool rt_warm_kernels(const pvvc_tuning& t, uint32_t* queue, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, queue);
     if (t.waves_per_patch == 2)
         hipLaunchKernelGGL(persistent_kernel<2>, dim3(1), dim3(128), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
-                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u);
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u, nullptr, nullptr);
     else if (t.waves_per_patch == 4)
         hipLaunchKernelGGL(persistent_kernel<4>, dim3(1), dim3(256), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
-                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u);
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u, nullptr, nullptr);
     else
         hipLaunchKernelGGL(persistent_kernel<1>, dim3(1), dim3(64), lds, s, nullptr, nullptr, 0u, queue, nullptr, sink,
-                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u);
+                           lds, nullptr, 0ull, 0u, nullptr, nullptr, 1u, 0u, nullptr, nullptr);
     return hip_ok(hipGetLastError()) && hip_ok(hipStreamSynchronize(s));
 }
 
@@ -2828,10 +2870,18 @@ int pvvc_decoder_create(pvvc_decoder** out, int max_patches, int max_points_per_
 int pvvc_decoder_destroy(pvvc_decoder* d) {
     if (!d) return PVVC_ERR_INVALID_ARG;
     hipStreamSynchronize(dec_stream(d));
+    if (d->aux) hipStreamSynchronize(d->aux);
     state_destroy(d->st);
     if (d->slab) hipFree(d->slab);
     if (d->queue) hipFree(d->queue);
     if (d->stage) hipFree(d->stage);
+    if (d->queue2) hipFree(d->queue2);
+    if (d->slab2) hipFree(d->slab2);
+    if (d->lane_coef) hipFree(d->lane_coef);
+    if (d->lane_status) hipFree(d->lane_status);
+    if (d->ev_fork) hipEventDestroy(d->ev_fork);
+    if (d->ev_join) hipEventDestroy(d->ev_join);
+    if (d->aux) hipStreamDestroy(d->aux);
     delete d;
     return PVVC_OK;
 }
@@ -2847,6 +2897,7 @@ int pvvc_decoder_set_tuning(pvvc_decoder* d, const pvvc_tuning* t) {
     if (t->wgs_per_cu) n.wgs_per_cu = t->wgs_per_cu;
     if (t->queue_chunk) n.queue_chunk = t->queue_chunk;
     if (t->priority_items) n.priority_items = t->priority_items;
+    if (t->lane_max_points) n.lane_max_points = t->lane_max_points < 0 ? 0 : t->lane_max_points;
     d->tuning = n;
     d->auto_small = false;
     return PVVC_OK;
@@ -2944,6 +2995,91 @@ int pvvc_batch_status(pvvc_batch* b, int* status, int n) {
     int k = std::min<int>(n, static_cast<int>(b->n_items));
     return hip_ok(hipMemcpy(status, b->status, 4ull * k, hipMemcpyDeviceToHost)) ? PVVC_OK : PVVC_ERR_HIP;
 }
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+bool grow_dev(T*& p, uint64_t& cap, uint64_t need, hipStream_t s) {
+    if (need <= cap) return true;
+    if (p) {
+        hipStreamSynchronize(s);
+        hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&p), need))) return false;
+    cap = need;
+    return true;
+}
+
+/* A segment whose queue tail [ns, count) holds slices of at most
+ * lane_max_points points.  On a second stream, forked from s: lane_rlgr_kernel
+ * decodes the tail's RLGR one slice per lane, then a persistent launch with
+ * those coefficients does the rest of their decode (own queue counter and
+ * workspaces).  On s, meanwhile, the scalar persistent launch decodes the
+ * head [0, ns).  The lane kernel is enqueued first so that its wavefronts
+ * (few registers, no LDS) are resident beside the scalar launch's.  The
+ * streams join before the finish pass of the deferred predictive slices. */
+int launch_segment_lanes(pvvc_decoder* d, pvvc_batch* b, const BatchDev& bd, const Segment& sg, uint32_t ns,
+                         const Sink& sink, const pvvc_tuning& t, const LaunchRes& res, uint32_t launch_id,
+                         hipStream_t s) {
+    const uint32_t nl = sg.count - ns;
+    if (!d->aux) {
+        if (!hip_ok(hipStreamCreateWithFlags(&d->aux, hipStreamNonBlocking)) ||
+            !hip_ok(hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming)) ||
+            !hip_ok(hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming)) ||
+            !hip_ok(hipMalloc(&d->queue2, 64)) || !hip_ok(hipMemset(d->queue2, 0, 64)))
+            return PVVC_ERR_HIP;
+    }
+    /* the tail's workspace bound and the second launch's slab */
+    uint64_t need2 = 256;
+    for (uint32_t q = sg.begin + ns; q < sg.begin + sg.count; ++q) {
+        const ItemDev& x = b->plan.items[b->plan.order[q]];
+        need2 = std::max(need2, item_ws_bound(x.n, x.geo_size ? x.geo_size : d->st.geo_cap));
+    }
+    need2 = (need2 + 255) & ~uint64_t(255);
+    pvvc_tuning t2 = t;
+    t2.waves_per_patch = 1;
+    const uint32_t grid2 = persistent_grid(t2, nl);
+    if (!grow_dev(d->slab2, d->slab2_total, need2 * grid2, s) ||
+        !grow_dev(d->lane_coef, d->lane_coef_bytes, 12ull * std::max<int64_t>(b->total_points, 1), s) ||
+        !grow_dev(d->lane_status, d->lane_status_bytes, 4ull * std::max<uint32_t>(b->n_items, 1), s))
+        return PVVC_ERR_HIP;
+    auto kev = g_ktiming.next();
+    hipEventRecord(kev.first, s);
+    hipEventRecord(d->ev_fork, s);
+    hipStreamWaitEvent(d->aux, d->ev_fork, 0);
+    hipLaunchKernelGGL(lane_rlgr_kernel, dim3((nl + 63) / 64), dim3(64), 0, d->aux, bd.items, bd.order + sg.begin + ns,
+                       nl, d->lane_coef, d->lane_status);
+    int rc = PVVC_OK;
+    if (ns) {
+        Segment head = sg;
+        head.count = ns;
+        head.dcount = 0;
+        LaunchRes r1 = res;
+        r1.timed = false;
+        rc = launch_segment(bd, head, sink, t, r1, launch_id, s);
+    }
+    Segment tail{sg.begin + ns, nl, 0, 0, need2};
+    LaunchRes r2{res.slots, d->slab2, need2, d->slab2_total, res.stage, d->queue2, 0u};
+    r2.pre_coef = d->lane_coef;
+    r2.pre_status = d->lane_status;
+    r2.timed = false;
+    if (rc == PVVC_OK) rc = launch_segment(bd, tail, sink, t2, r2, launch_id, d->aux);
+    hipEventRecord(d->ev_join, d->aux);
+    hipStreamWaitEvent(s, d->ev_join, 0);
+    hipEventRecord(kev.second, s);
+    if (rc == PVVC_OK && sg.dcount)
+        hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, bd.items, bd.defer + sg.dbegin, sg.dcount,
+                           res.slots, sink, res.stage, bd.status);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
 
 int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, float* xyz, uint8_t* rgb,
                       Points* points) {
@@ -3051,9 +3187,17 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
     for (auto& sg : b->segs) {
         if (!sg.count) continue;
         const uint32_t id = ++d->launch_id == 0 ? ++d->launch_id : d->launch_id;
+        const pvvc_tuning tt = segment_tuning(d->tuning, d->auto_small, sg.count);
+        /* slices of at most lane_max_points points: the tail of the
+         * length-sorted queue */
+        uint32_t ns = sg.count;
+        if (tt.lane_max_points > 0)
+            while (ns > 0 && b->plan.items[b->plan.order[sg.begin + ns - 1]].n <= static_cast<uint32_t>(tt.lane_max_points))
+                --ns;
         auto ev = g_timing.next();
         hipEventRecord(ev.first, s);
-        const int rc = launch_segment(bd, sg, sink, segment_tuning(d->tuning, d->auto_small, sg.count), res, id, s);
+        const int rc = ns == sg.count ? launch_segment(bd, sg, sink, tt, res, id, s)
+                                      : launch_segment_lanes(d, b, bd, sg, ns, sink, tt, res, id, s);
         hipEventRecord(ev.second, s);
         if (rc != PVVC_OK) return rc;
     }
@@ -3131,8 +3275,9 @@ int pvvc_decode_batch_noalloc(pvvc_decoder* d, pvvc_batch* b, int output, float*
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, queue);
     if (!hip_ok(hipGetLastError())) return PVVC_ERR_HIP;
     const BatchDev bd{b->items, b->order, b->defer, b->status};
-    const LaunchRes res{d->st.slots, base + o_slab, need, slab_total, reinterpret_cast<float*>(base + o_stage),
-                        queue, grid_cap};
+    LaunchRes res{d->st.slots, base + o_slab, need, slab_total, reinterpret_cast<float*>(base + o_stage), queue,
+                  grid_cap};
+    res.timed = false;
     for (auto& sg : b->segs) {
         if (!sg.count) continue;
         const uint32_t id = ++d->launch_id == 0 ? ++d->launch_id : d->launch_id;

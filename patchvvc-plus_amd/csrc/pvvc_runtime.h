@@ -64,7 +64,10 @@ struct LaunchRes {
     uint64_t slab_total;   /* bytes of the whole slab */
     float* stage;
     uint32_t* queue;
-    uint32_t grid_cap;     /* at most this many workgroups (0: no cap); a capped launch is untimed */
+    uint32_t grid_cap;     /* at most this many workgroups (0: no cap) */
+    const float* pre_coef = nullptr;  /* coefficients decoded by lane_rlgr_kernel (decode_slice) */
+    const int* pre_status = nullptr;
+    bool timed = true;     /* record the launch's HIP events for pvvc_last_decode_kernel_times */
 };
 
 /* Workgroups of a persistent launch of `count` items. */

@@ -406,8 +406,10 @@ class Decoder:
         except Exception:
             pass
 
-    def set_tuning(self, waves_per_patch=0, lds_bytes=0, wgs_per_cu=0, queue_chunk=0, priority_items=0) -> None:
-        t = Tuning(waves_per_patch, lds_bytes, wgs_per_cu, queue_chunk, priority_items)
+    def set_tuning(self, waves_per_patch=0, lds_bytes=0, wgs_per_cu=0, queue_chunk=0, priority_items=0,
+                   lane_max_points=0) -> None:
+        """Zero fields keep their value; lane_max_points=-1 turns lane decoding off."""
+        t = Tuning(waves_per_patch, lds_bytes, wgs_per_cu, queue_chunk, priority_items, lane_max_points)
         _check(hip_lib().pvvc_decoder_set_tuning(self.handle, C.byref(t)), "set_tuning")
 
     def set_stream(self, stream: "Stream | None") -> None:

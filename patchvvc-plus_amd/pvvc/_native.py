@@ -33,7 +33,7 @@ class DropinFrame(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("waves_per_patch", C.c_int), ("lds_bytes", C.c_int), ("wgs_per_cu", C.c_int),
-                ("queue_chunk", C.c_int), ("priority_items", C.c_int)]
+                ("queue_chunk", C.c_int), ("priority_items", C.c_int), ("lane_max_points", C.c_int)]
 
 
 class ServerConfig(C.Structure):
