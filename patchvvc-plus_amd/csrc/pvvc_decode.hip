@@ -149,8 +149,12 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  *  - the raw Golomb value u is written by v_writelane into lane
  *    (nrel - lim) mod 64 of `acc`, selected by the low bits of `neg`
  *    (Unsign2Sign is applied when the chunk is stored);
- *  - the window takes 32 more bits when fewer than 33 remain; the borrow of
- *    nbx -= length says so.
+ *  - the window takes 32 more bits when fewer than 33 remain.  The code's
+ *    length is subtracted from nbx as soon as it is known: no borrow (the
+ *    common case) means the code lies in the window's first 32 bits and no
+ *    refill follows, so that one test stands for both; a borrow leaves the
+ *    line for a block that tests for a code over 32 bits (an exit, nbx
+ *    restored), then finishes the symbol and refills.
  * The k = 0 and run cases are two loops, each unrolled four times (k = 0:
  * 10, 16, 50, 54; run: 30, 40, 60, 70) and handing over to the other when
  * k_P crosses 3, so that the common path falls through: a taken scalar
@@ -185,16 +189,18 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * take them on the way out (85-88).  A non-zero k = 0 symbol's -1 is not
  * clamped: k_P - 4 may fall below -4 in the k = 0 loop (by at most one chunk
  * of symbols) and is clamped where it is read, in the zero block and at the
- * exits.  Refills (11/13/51/55, 20, 33/43/63/73,
- * 38) are out of line.
+ * exits.  Symbols that end in a refill finish out of line (11/13/51/55, 18,
+ * 33/43/63/73, 38).
  * A code is decoded here only when it lies entirely in the first 32 bits of
- * the window, so every check precedes any state change.  The loop returns
+ * the window; every exit leaves the state of the last whole symbol (the
+ * early nbx and m0 updates are undone on the way out: 19, 97, 98 and
+ * 37/47/67/77).  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
  * of `cur`, a unary prefix of 32+ or a code longer than 32 bits, more than 6
  * one-bits in a run prefix or k > 24 after them, or a zero run reaching the
  * chunk end.  s80-s93 are scratch.  One SGPR written by a scalar
  * instruction and then read as a v_readlane lane select needs 4 wait states
- * (here: four instructions and an s_nop 1). */
+ * (here: three instructions and an s_nop 0). */
 __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint32_t& wpos, uint32_t& kRP,
                                                uint32_t& kP, uint32_t& neg, uint32_t cur, int32_t& acc) {
     asm volatile(
@@ -207,8 +213,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 11f\n\t"
         "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
@@ -220,8 +226,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
-        "s_cbranch_scc1 11f\n\t"
         "12:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -231,8 +235,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 13f\n\t"
         "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
@@ -244,8 +248,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
-        "s_cbranch_scc1 13f\n\t"
         "14:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -255,8 +257,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 51f\n\t"
         "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
@@ -268,8 +270,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
-        "s_cbranch_scc1 51f\n\t"
         "52:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -279,8 +279,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, %[win]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_cbranch_scc1 55f\n\t"
         "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
@@ -292,19 +292,29 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "v_writelane_b32 %[acc], s89, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
-        "s_cbranch_scc1 55f\n\t"
         "56:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
         "s_branch 10b\n\t"
         "11:\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 19f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s87\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
+        "s_cbranch_scc0 18f\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
@@ -312,12 +322,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 12b\n\t"
         "13:\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 19f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s87\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
+        "s_cbranch_scc0 18f\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
@@ -325,12 +347,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 14b\n\t"
         "51:\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 19f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s87\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
+        "s_cbranch_scc0 18f\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
@@ -338,25 +372,38 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 52b\n\t"
         "55:\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 19f\n\t"
+        "s_lshl_b64 s[80:81], %[win], s84\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s87\n\t"
+        "s_lshl_b32 s89, s84, s85\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_or_b32 s89, s89, s82\n\t"
+        "s_cbranch_scc0 18f\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 56b\n\t"
-        "20:\n\t"
+        "18:\n\t"
+        "s_sub_i32 s90, s90, 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 88f\n\t"
+        "s_cbranch_scc0 88f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
@@ -366,8 +413,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "17:\n\t"
         "s_sub_i32 s90, s90, 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
-        "s_cbranch_scc1 20b\n\t"
         "21:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 86f\n\t"
@@ -390,8 +435,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 37f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 33f\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
@@ -403,8 +448,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 33f\n\t"
         "34:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 85f\n\t"
@@ -426,8 +469,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 47f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 43f\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
@@ -439,8 +482,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 43f\n\t"
         "44:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 85f\n\t"
@@ -462,8 +503,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 67f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 63f\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
@@ -475,8 +516,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 63f\n\t"
         "64:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 85f\n\t"
@@ -498,8 +537,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 77f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 73f\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
@@ -511,8 +550,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 73f\n\t"
         "74:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 85f\n\t"
@@ -520,12 +557,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 10b\n\t"
         "s_branch 30b\n\t"
         "33:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 97f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 87f\n\t"
+        "s_cbranch_scc0 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
@@ -533,31 +582,49 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 34b\n\t"
         "43:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 97f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 87f\n\t"
+        "s_cbranch_scc0 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 44b\n\t"
-        "37:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
-        "s_branch 9f\n\t"
-        "47:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
-        "s_branch 9f\n\t"
         "63:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 97f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 87f\n\t"
+        "s_cbranch_scc0 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
@@ -565,18 +632,36 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 64b\n\t"
         "73:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 97f\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 87f\n\t"
+        "s_cbranch_scc0 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 74b\n\t"
+        "37:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_branch 9f\n\t"
+        "47:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_branch 9f\n\t"
         "67:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
@@ -618,8 +703,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_add_u32 s86, s86, 1\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 9f\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_cbranch_scc1 38f\n\t"
         "s_mov_b32 m0, s83\n\t"
         "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
         "s_sub_u32 s88, 63, s89\n\t"
@@ -633,8 +718,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[acc], s88, m0\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 38f\n\t"
         "39:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -642,18 +725,42 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 10b\n\t"
         "s_branch 30b\n\t"
         "38:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 98f\n\t"
+        "s_mov_b32 m0, s83\n\t"
+        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
+        "s_sub_u32 s88, 63, s89\n\t"
+        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
+        "s_lshl_b64 %[win], %[win], s86\n\t"
+        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_or_b32 s88, s88, s90\n\t"
+        "s_add_u32 s88, s88, 1\n\t"
+        "s_sub_u32 %[kP], s87, 5\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
-        "s_cmp_eq_u32 s88, 0\n\t"
-        "s_cbranch_scc1 7f\n\t"
+        "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 1\n\t"
+        "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
         "s_or_b64 %[win], %[win], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 39b\n\t"
+        "19:\n\t"
+        "s_add_u32 %[nbx], %[nbx], s87\n\t"
+        "s_branch 9f\n\t"
+        "97:\n\t"
+        "s_add_u32 %[nbx], %[nbx], s86\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_branch 9f\n\t"
+        "98:\n\t"
+        "s_add_u32 %[nbx], %[nbx], s86\n\t"
+        "s_branch 9f\n\t"
         "85:\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_branch 9f\n\t"
@@ -3553,10 +3660,11 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
     static const int kLds[] = {4096, 8192, 16384, 24576, 32768, 49152, 65536};
     static const int kWgs[] = {2, 4, 8, 12, 16, 20, 24, 32};
     static const int kChunk[] = {1, 2, 4, 8};
+    static const int kLane[] = {0, 64, 256, 1000}; /* lane_max_points: 0 = the scalar decoder only */
     const bool persistent = mode == PVVC_MODE_PERSISTENT;
-    std::map<std::tuple<int, int, int, int>, double> seen;
+    std::map<std::tuple<int, int, int, int, int>, double> seen;
     auto measure = [&](const pvvc_tuning& t, double& ms) -> int {
-        auto key = std::make_tuple(t.waves_per_patch, t.lds_bytes, t.wgs_per_cu, t.queue_chunk);
+        auto key = std::make_tuple(t.waves_per_patch, t.lds_bytes, t.wgs_per_cu, t.queue_chunk, t.lane_max_points);
         auto f = seen.find(key);
         if (f != seen.end()) {
             ms = f->second;
@@ -3583,14 +3691,17 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
     if (rc != PVVC_OK) return rc;
     for (int pass = 0; pass < 3; ++pass) {
         const pvvc_tuning start = cur;
-        for (int dim = 0; dim < 4; ++dim) {
+        for (int dim = 0; dim < 5; ++dim) {
             if (!persistent && dim >= 2) break; /* grid shape is the slice count there */
-            const int* vals = dim == 0 ? kWaves : dim == 1 ? kLds : dim == 2 ? kWgs : kChunk;
+            const int* vals = dim == 0 ? kWaves : dim == 1 ? kLds : dim == 2 ? kWgs : dim == 3 ? kChunk : kLane;
             const int nv = dim == 0 ? 3 : dim == 1 ? 7 : dim == 2 ? 8 : 4;
             for (int k = 0; k < nv; ++k) {
                 pvvc_tuning t = cur;
-                int* field = dim == 0 ? &t.waves_per_patch : dim == 1 ? &t.lds_bytes : dim == 2 ? &t.wgs_per_cu
-                                                                                        : &t.queue_chunk;
+                int* field = dim == 0   ? &t.waves_per_patch
+                             : dim == 1 ? &t.lds_bytes
+                             : dim == 2 ? &t.wgs_per_cu
+                             : dim == 3 ? &t.queue_chunk
+                                        : &t.lane_max_points;
                 if (*field == vals[k]) continue;
                 *field = vals[k];
                 double ms = 0;

@@ -8,7 +8,7 @@ encoder makes from symbol vectors built to reach:
   run prefixes of one to many one-bits (past the loop's 6, into the general
   path);
 * Golomb escapes (32 one-bits and a raw 32-bit value) and codes longer than
-  32 bits (large k_R after large values);
+  32 bits (large k_R after large values), in run mode and in k = 0 mode;
 * frequent k = 0 / run mode changes, zero and non-zero k = 0 symbols;
 * plain small random values.
 
@@ -67,10 +67,17 @@ def _symbols(kind, L, rng):
         return s
     if kind == "small":
         return rng.integers(-3, 4, L)
+    if kind == "k0_escape":
+        # k = 0 mode (non-zero symbols keep k_P low) with sudden large values:
+        # unary prefixes of 32+ (Golomb escapes) and codes over 32 bits
+        s = rng.integers(1, 4, L) * rng.choice([-1, 1], L)
+        m = rng.random(L) < 0.02
+        s[m] = rng.integers(1 << 12, 1 << 30, m.sum()) * rng.choice([-1, 1], m.sum())
+        return s
     raise ValueError(kind)
 
 
-KINDS = ["sparse", "bursty", "huge", "mixed", "ramp", "long_runs", "alternate", "small"]
+KINDS = ["sparse", "bursty", "huge", "mixed", "ramp", "long_runs", "alternate", "small", "k0_escape"]
 
 
 def _stream(pvvc, oracle, seed=12, per_kind=3):
