@@ -1589,14 +1589,16 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
     }
 }
 
-/* RLGR of short slices, one lane per slice (rlgr_decode_lane); the rest of
- * their decode runs in persistent_kernel with these coefficients.  Wavefront
- * w takes list[64 w, 64 w + 64): it zeroes their coefficient ranges together
- * (coalesced), then each lane decodes its own slice into coef_all at
- * 3 * stage_off and leaves the status in lane_status[item]. */
+/* RLGR of short slices, one lane per slice (rlgr_decode_lane_from); the
+ * rest of their decode runs in persistent_kernel with these coefficients.
+ * Wavefront w takes list[64 w, 64 w + 64): it zeroes their coefficient
+ * ranges together (coalesced), then each lane decodes its own slice into
+ * coef_all at 3 * stage_off, reading its colour bytes through an LDS ring
+ * (LaneRing), and leaves the status in lane_status[item]. */
 __global__ __launch_bounds__(64) void lane_rlgr_kernel(const ItemDev* __restrict__ items,
                                                        const uint32_t* __restrict__ list, uint32_t count,
                                                        float* __restrict__ coef_all, int* __restrict__ lane_status) {
+    __shared__ uint32_t ring[(kLaneRing + 1u) * 64u];
     const uint32_t base = blockIdx.x * 64u;
     if (base >= count) return;
     const uint32_t lane = threadIdx.x;
@@ -1611,7 +1613,14 @@ __global__ __launch_bounds__(64) void lane_rlgr_kernel(const ItemDev* __restrict
     if (lane >= cnt) return;
     const uint32_t i = list[base + lane];
     const ItemDev* it = items + i;
-    lane_status[i] = rlgr_decode_lane(it->col, it->col_size, 3u * it->n, it->qp, coef_all + 3ull * it->stage_off);
+    /* a slice without colour bytes points the reader at its own descriptor,
+     * which is readable */
+    const uint8_t* src = it->col_size ? it->col : reinterpret_cast<const uint8_t*>(it);
+    const uintptr_t a = reinterpret_cast<uintptr_t>(src);
+    const uint32_t skip = 8u * static_cast<uint32_t>(a & 3u);
+    const uint32_t end = skip + 8u * it->col_size;
+    const LaneRing r{reinterpret_cast<const uint32_t*>(a & ~uintptr_t(3)), end ? (end - 1u) / 32u : 0u, 0u, ring + lane};
+    lane_status[i] = rlgr_decode_lane_from(r, skip, it->col_size, 3u * it->n, it->qp, coef_all + 3ull * it->stage_off);
 }
 
 /* Zeroes a queue's counters once, after allocation. */

@@ -8,6 +8,8 @@ built for the host and checked against the oracle's RLGR decoder
   bytes after the end that must not be read: coefficients equal
   float(int64(symbol) * qp), statuses agree.
 * Symbols up to the 32-bit range (Golomb escapes).
+* Both dword sources: straight from memory, and through the LDS ring's
+  indexing (rounds of 16 dwords, wrap-around, the spare row).
 * Random bytes: every status class.  The GPU decoders reject a stored value
   outside 32 bits (SYMBOL_RANGE) where the oracle's 64-bit decoder carries
   on; everything else agrees with the oracle exactly.
@@ -35,15 +37,21 @@ def lane(tmp_path_factory):
     subprocess.run([HIPCC, "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++17", "-shared", "-fPIC",
                     str(src), "-o", str(so)], check=True, capture_output=True)
     lib = C.CDLL(str(so))
-    lib.lane_decode.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
-    lib.lane_decode.restype = C.c_int
+    for fn in (lib.lane_decode, lib.lane_decode_ring):
+        fn.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+        fn.restype = C.c_int
 
     def decode(data: bytes, total: int, qp: int, align: int):
         buf = np.full(len(data) + 12, 0xFF, np.uint8)  # bytes past the end read as ones would break runs
         buf[align:align + len(data)] = np.frombuffer(data, np.uint8)
-        out = np.zeros(max(total, 1), np.float32)
-        st = lib.lane_decode(buf.ctypes.data + align, len(data), total, qp, out.ctypes.data)
-        return out[:total], st
+        res = []
+        for fn in (lib.lane_decode, lib.lane_decode_ring):
+            out = np.zeros(max(total, 1), np.float32)
+            st = fn(buf.ctypes.data + align, len(data), total, qp, out.ctypes.data)
+            res.append((out[:total], st))
+        (o1, s1), (o2, s2) = res
+        assert s1 == s2 and (s1 != 0 or np.array_equal(o1, o2)), "memory and ring readers differ"
+        return o1, s1
     return decode
 
 
