@@ -73,7 +73,7 @@ def measure_counters(args) -> dict | None:
     leg = me + ["--only", "batched", "--steps", str(args.gops), "--warmup", "1", "--gops", str(args.gops),
                 "--no-cpu", "--no-traffic", "--no-encode", "--no-quality", "--no-c2", "--preset", args.preset,
                 "--seed", str(args.seed)]
-    for k in ("waves", "lds", "wgs_per_cu", "queue_chunk", "priority_items"):
+    for k in ("waves", "lds", "wgs_per_cu", "queue_chunk", "priority_items", "lane_max_points"):
         v = getattr(args, k)
         if v:
             leg += ["--" + k.replace("_", "-"), str(v)]
@@ -237,6 +237,8 @@ def main() -> int:
     ap.add_argument("--wgs-per-cu", type=int, default=0)
     ap.add_argument("--queue-chunk", type=int, default=0)
     ap.add_argument("--priority-items", type=int, default=0)
+    ap.add_argument("--lane-max-points", type=int, default=0,
+                    help="decode the RLGR of slices up to this many points one slice per lane (default off)")
     ap.add_argument("--autotune", action="store_true", help="run the decoder autotuner on the first GoP before timing")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
     ap.add_argument("--no-serving", action="store_true", help="skip the multi-stream serving leg")
@@ -297,7 +299,8 @@ def main() -> int:
     run_p = args.only in ("all", "perframe")
     dec = pvvc.Decoder(n_patches, max_pts)
     tun = {k: v for k, v in dict(waves_per_patch=args.waves, lds_bytes=args.lds, wgs_per_cu=args.wgs_per_cu,
-                                 queue_chunk=args.queue_chunk, priority_items=args.priority_items).items() if v}
+                                 queue_chunk=args.queue_chunk, priority_items=args.priority_items,
+                                 lane_max_points=args.lane_max_points).items() if v}
     if tun:
         dec.set_tuning(**tun)
     batches = [dec.batch(frames[g * gop:(g + 1) * gop]) for g in range(args.gops)]
