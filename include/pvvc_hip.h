@@ -159,6 +159,48 @@ void pvvc_frame_meta(const pvvc_frame* frame, int* index, uint8_t* type, uint32_
 const uint8_t* pvvc_frame_geometry(const pvvc_frame* frame, uint32_t slice);
 const uint8_t* pvvc_frame_color(const pvvc_frame* frame, uint32_t slice);
 
+/* ------------------------------------------------------------------------ */
+/* .slice files (host): one patch slice per file, the layout of io::SaveSlice */
+/* / io::LoadSlice (src/io/slice_io.cc:20-206):                              */
+/*   u8 type; i32 timestamp; i32 index; u64 size; u8 qp;                     */
+/*   f32 mv[16] column-major (Eigen::Matrix4f storage);                      */
+/*   if not predictive (type bit 1): u64 geometry size > 0, geometry bytes;  */
+/*   if not skip (type bit 2): u64 colour size > 0, colour bytes.            */
+/* Codes: 1 wrong file name ("*.slice" in, "*.frame" out), 2 no such file    */
+/* (or no slice file in the directory), 3 permission denied, 4 other open    */
+/* error or not a directory, 5 short read, 6 zstd failure, 7 type without   */
+/* the valid bit, 8 empty geometry or colour blob, 9 no slices, 10 write     */
+/* error.                                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct pvvc_slice_desc {
+    uint8_t type;
+    int32_t timestamp;
+    int32_t index;
+    uint64_t size;           /* points */
+    uint8_t qp;
+    float mv[16];            /* row-major, as in a .frame; stored transposed */
+    const uint8_t* geometry; /* written unless the slice is predictive */
+    uint64_t geometry_size;
+    const uint8_t* color;    /* written unless the slice is a skip */
+    uint64_t color_size;
+} pvvc_slice_desc;
+
+/* io::SaveSlice (slice_io.cc:20-120): returns the bytes written, or minus a
+ * code above. */
+int64_t pvvc_save_slice(const char* path, const pvvc_slice_desc* slice);
+/* io::LoadSlice of each path (slice_io.cc:124-206), then the frame that
+ * io::ChangeSliceToFrame would write for them (slice_io.cc:208-315: slices
+ * ordered by patch index, stable among equal indices; the first ordered
+ * slice's timestamp), parsed and zstd-unwrapped as pvvc_parse_frame does.
+ * The reference decoder harness reads slices this way
+ * (PVVCDecompression::LoadSlices, src/codec/pvvc_decompression.cc:34-84). */
+int pvvc_load_slices(const char* const* paths, int n, pvvc_frame** out);
+/* Writes that frame as a .frame file (the bytes ChangeSliceToFrame writes). */
+int pvvc_slices_to_frame_file(const char* const* paths, int n, const char* frame_path);
+/* io::ChangeSliceToFrame (slice_io.cc:208-315): every "*_time_<t>_slice_<i>.slice"
+ * file of input_dir into one .frame file. */
+int pvvc_change_slice_to_frame(const char* input_dir, const char* frame_path);
+
 /* Stages a frame for launch_cudaProcess the way test/test_decode_only.cc:
  * 97-123 does, but with one packed host->device copy: fills the device
  * arrays of the drop-in call.  `dev` must come from pvvc_dropin_alloc. */

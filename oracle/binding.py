@@ -60,12 +60,31 @@ def lib() -> C.CDLL:
     sig("orc_raht_forward", C.c_int, vp, C.c_uint32, vp, C.c_uint32, vp)
     sig("orc_raht_inverse", C.c_int, vp, C.c_uint32, C.c_uint32, vp, vp)
     sig("orc_color_encode", C.c_int64, vp, C.c_uint32, vp, C.c_uint32, C.c_uint8, vp, vp, C.c_int64)
+    sig("orc_save_slice", C.c_int64, C.c_char_p, C.c_uint8, C.c_int, C.c_int, C.c_uint64, C.c_uint8, vp, vp,
+        C.c_uint64, vp, C.c_uint64)
+    sig("orc_slices_to_frame", C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_char_p)
     _lib = L
     return L
 
 
 def _buf(b: bytes):
     return (C.c_uint8 * max(1, len(b))).from_buffer_copy(b if b else b"\0")
+
+
+def save_slice(path: str, type_: int, timestamp: int, index: int, size: int, qp: int, mv, geo: bytes | None,
+               col: bytes | None) -> int:
+    """SaveSlice (slice_io.cc:20-120); mv is 4x4 row-major.  Bytes written or minus a code."""
+    m = np.ascontiguousarray(np.asarray(mv, np.float32).reshape(16))
+    g = _buf(geo or b"") if geo else None
+    c = _buf(col or b"") if col else None
+    return lib().orc_save_slice(str(path).encode(), type_, timestamp, index, size, qp, m.ctypes.data, g,
+                                len(geo or b""), c, len(col or b""))
+
+
+def slices_to_frame(paths, out: str) -> int:
+    """LoadSlice of each path, then ChangeSliceToFrame's .frame file."""
+    arr = (C.c_char_p * max(1, len(paths)))(*[str(p).encode() for p in paths])
+    return lib().orc_slices_to_frame(arr, len(paths), str(out).encode())
 
 
 def rlgr_encode(sym) -> bytes:

@@ -19,7 +19,8 @@
  * brute-force recursive expansion; the inverse RAHT by the two-leaf closed
  * form, the constant-colour DC identity, and forward->inverse identity; the
  * patch motion by exact closed forms of non-symmetric integer / power-of-two
- * matrices; the container by hand-packed byte fixtures; the quality metric
+ * matrices; the container and the .slice files by hand-packed byte
+ * fixtures (a non-symmetric motion matrix fixes the storage orders); the quality metric
  * (MSE::Compute) by brute-force neighbours and closed forms.  The reference itself holds no
  * golden vectors for this path.
  *
@@ -39,6 +40,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cerrno>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -48,6 +50,7 @@
 #include <memory>
 #include <mutex>
 #include <queue>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -745,6 +748,118 @@ int load_frame(const char* path, Frame& f) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* .slice files: io::SaveSlice / io::LoadSlice (src/io/slice_io.cc:20-206)   */
+/* and io::ChangeSliceToFrame with the Frame(slices) constructor            */
+/* (slice_io.cc:208-315, include/common/common.h:378-412).  The reference   */
+/* throws; here each exception is a code (the product's numbering):        */
+/* WRONG_FILE_FORMAT 1, FILE_NOT_EXIST 2, PERMISSION_DENIED 3,              */
+/* UNEXPECTED_FILE_ERROR / INVALID_DIR 4, FILE_READ_ERROR 5, BAD_SLICE 7,   */
+/* EMPTY_RESULT (zero blob size) 8, EMPTY_RESULT (no slices) 9,             */
+/* FILE_WRITE_ERROR 10.                                                      */
+/* ------------------------------------------------------------------------ */
+struct OrcSlice {
+    int timestamp = -1;
+    int index = -1;
+    uint8_t type = 0;
+    float mv[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}}; /* mv[row][col] */
+    uint64_t size = 0;
+    uint8_t qp = 1;
+    bool geometry = false, color = false; /* shared_ptr present */
+    std::vector<uint8_t> geo, col;
+};
+
+bool ends_with(const std::string& s, const std::string& t) {
+    return s.size() >= t.size() && s.compare(s.size() - t.size(), t.size(), t) == 0;
+}
+
+/* LoadSlice, slice_io.cc:124-206 */
+int load_slice(const std::string& name, OrcSlice& sl) {
+    if (!ends_with(name, ".slice")) return 1;
+    std::FILE* fp = std::fopen(name.c_str(), "rb");
+    if (!fp) return errno == ENOENT ? 2 : (errno == EACCES ? 3 : 4);
+    std::fseek(fp, 0, SEEK_END);
+    const long total = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    int rc = 0;
+    auto one = [&](void* dst, size_t n) {
+        if (rc) return false;
+        if (std::fread(dst, 1, n, fp) != n) rc = 5;
+        return rc == 0;
+    };
+    sl = OrcSlice{};
+    if (one(&sl.type, 1) && !(sl.type & 0x01)) rc = 7; /* PVVC_SLICE_TYPE_VALID */
+    one(&sl.timestamp, sizeof(int));
+    one(&sl.index, sizeof(int));
+    one(&sl.size, sizeof(uint64_t));
+    one(&sl.qp, 1);
+    float cm[16];
+    if (one(cm, sizeof(cm)))
+        for (int k = 0; k < 16; ++k) sl.mv[k % 4][k / 4] = cm[k]; /* Eigen storage is column-major */
+    auto blob = [&](std::vector<uint8_t>& v, bool& present) {
+        uint64_t n = 0;
+        if (!one(&n, sizeof(uint64_t))) return;
+        if (n == 0) {
+            rc = 8;
+            return;
+        }
+        const long at = std::ftell(fp);
+        if (at < 0 || static_cast<uint64_t>(total - at) < n) {
+            rc = 5;
+            return;
+        }
+        v.resize(n);
+        if (one(v.data(), n)) present = true;
+    };
+    if (!rc && !(sl.type & 0x02)) blob(sl.geo, sl.geometry); /* not PREDICT */
+    if (!rc && !(sl.type & 0x04)) blob(sl.col, sl.color);    /* not SKIP */
+    std::fclose(fp);
+    return rc;
+}
+
+/* ChangeSliceToFrame's writing part, slice_io.cc:238-310 */
+int write_frame_of(std::vector<OrcSlice> slices, const std::string& out) {
+    if (!ends_with(out, ".frame")) return 1;
+    if (slices.empty()) return 9;
+    std::stable_sort(slices.begin(), slices.end(),
+                     [](const OrcSlice& x, const OrcSlice& y) { return x.index < y.index; });
+    std::vector<uint8_t> o;
+    auto w = [&](const void* p, size_t n) {
+        const auto* b = static_cast<const uint8_t*>(p);
+        o.insert(o.end(), b, b + n);
+    };
+    const int ts = slices.front().timestamp; /* Frame(slices): the first slice's */
+    const uint32_t cnt = static_cast<uint32_t>(slices.size());
+    w(&ts, sizeof(int));
+    w(&cnt, sizeof(uint32_t));
+    for (const auto& sl : slices) w(&sl.index, sizeof(int));
+    for (const auto& sl : slices) w(&sl.type, 1);
+    for (const auto& sl : slices) {
+        const uint32_t sz = static_cast<uint32_t>(sl.size);
+        w(&sz, sizeof(uint32_t));
+    }
+    for (const auto& sl : slices) w(&sl.qp, 1);
+    for (const auto& sl : slices)
+        for (int row = 0; row < 4; ++row)
+            for (int col = 0; col < 4; ++col) w(&sl.mv[row][col], sizeof(float));
+    for (const auto& sl : slices) {
+        const uint32_t g = sl.geometry ? static_cast<uint32_t>(sl.geo.size()) : 0u;
+        w(&g, sizeof(uint32_t));
+    }
+    for (const auto& sl : slices)
+        if (sl.geometry) w(sl.geo.data(), sl.geo.size());
+    for (const auto& sl : slices) {
+        const uint32_t c = sl.color ? static_cast<uint32_t>(sl.col.size()) : 0u;
+        w(&c, sizeof(uint32_t));
+    }
+    for (const auto& sl : slices)
+        if (sl.color) w(sl.col.data(), sl.col.size());
+    std::FILE* fp = std::fopen(out.c_str(), "wb");
+    if (!fp) return errno == ENOENT ? 2 : (errno == EACCES ? 3 : 4);
+    const bool ok = std::fwrite(o.data(), 1, o.size(), fp) == o.size();
+    return (std::fclose(fp) == 0 && ok) ? 0 : 10;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Quality metric: symmetric nearest-neighbour MSE, MSE::Compute            */
 /* (src/registration/registration_base.cc:168-223).                         */
 /*                                                                          */
@@ -1285,6 +1400,55 @@ int orc_cloud_mse(const float* pxyz, const uint8_t* prgb, int64_t np, const floa
     if (nn_p) std::memcpy(nn_p, ip.data(), sizeof(int64_t) * np);
     if (nn_q) std::memcpy(nn_q, iq.data(), sizeof(int64_t) * nq);
     return 0;
+}
+
+/* SaveSlice (slice_io.cc:20-120): mv_rows is row-major.  Returns the bytes
+ * written (the reference's stream_size) or minus a code. */
+int64_t orc_save_slice(const char* path, uint8_t type, int timestamp, int index, uint64_t size, uint8_t qp,
+                       const float* mv_rows, const uint8_t* geo, uint64_t geo_size, const uint8_t* col,
+                       uint64_t col_size) {
+    const std::string name = path ? path : "";
+    if (!ends_with(name, ".slice")) return -1;
+    if (!(type & 0x01)) return -7;
+    const bool g = !(type & 0x02), c = !(type & 0x04);
+    if ((g && (!geo || !geo_size)) || (c && (!col || !col_size))) return -8;
+    std::FILE* fp = std::fopen(name.c_str(), "wb");
+    if (!fp) return errno == ENOENT ? -2 : (errno == EACCES ? -3 : -4);
+    int64_t n = 0;
+    bool ok = true;
+    auto w = [&](const void* p, size_t k) {
+        ok = ok && std::fwrite(p, 1, k, fp) == k;
+        n += static_cast<int64_t>(k);
+    };
+    w(&type, 1);
+    w(&timestamp, sizeof(int));
+    w(&index, sizeof(int));
+    w(&size, sizeof(uint64_t));
+    w(&qp, 1);
+    float cm[16];
+    for (int k = 0; k < 16; ++k) cm[k] = mv_rows[(k % 4) * 4 + k / 4];
+    w(cm, sizeof(cm));
+    if (g) {
+        w(&geo_size, sizeof(uint64_t));
+        w(geo, geo_size);
+    }
+    if (c) {
+        w(&col_size, sizeof(uint64_t));
+        w(col, col_size);
+    }
+    if (std::fclose(fp) != 0 || !ok) return -10;
+    return n;
+}
+
+/* LoadSlice of each path, then ChangeSliceToFrame's frame written to out. */
+int orc_slices_to_frame(const char* const* paths, int n, const char* out) {
+    std::vector<OrcSlice> slices;
+    for (int i = 0; i < n; ++i) {
+        OrcSlice sl;
+        if (int rc = load_slice(paths[i] ? paths[i] : "", sl)) return rc;
+        slices.push_back(std::move(sl));
+    }
+    return write_frame_of(std::move(slices), out ? out : "");
 }
 
 }  // extern "C"

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <filesystem>
 #include <functional>
 #include <mutex>
 #include <new>
@@ -330,6 +331,204 @@ const uint8_t* pvvc_frame_geometry(const pvvc_frame* f, uint32_t s) {
 const uint8_t* pvvc_frame_color(const pvvc_frame* f, uint32_t s) {
     if (!f || s >= f->slice_cnt || !f->col_size[s]) return nullptr;
     return f->data.data() + f->col_off[s];
+}
+
+/* ------------------------------------------------------------------------
+ * .slice files: one patch slice each (io::SaveSlice / io::LoadSlice,
+ * src/io/slice_io.cc:20-206), and their assembly into a frame
+ * (io::ChangeSliceToFrame, :208-315):
+ *
+ *   u8 type; i32 timestamp; i32 index; u64 size; u8 qp;
+ *   f32 mv[16] (Eigen::Matrix4f storage: column-major);
+ *   if not predictive: u64 geometry size (> 0), geometry bytes;
+ *   if not skip:       u64 colour size (> 0), colour bytes.
+ *
+ * Bytes after the colour blob are ignored (the reference warns).  Slices
+ * are ordered by patch index (a stable order among equal indices, which the
+ * reference leaves unspecified); the frame's timestamp is the first ordered
+ * slice's; sizes are written as u32; mv is written row-major.
+ * ------------------------------------------------------------------------ */
+}  // extern "C"
+
+namespace {
+
+struct SliceRec {
+    uint8_t type = 0;
+    int timestamp = -1, index = -1;
+    uint64_t size = 0;
+    uint8_t qp = 1;
+    float mv_cm[16] = {};  /* column-major, as stored */
+    bool has_geo = false, has_col = false;
+    std::vector<uint8_t> geo, col;
+};
+
+int open_code(int e) { return e == ENOENT ? 2 : e == EACCES ? 3 : 4; }
+
+/* LoadSlice (slice_io.cc:124-206) with its exceptions as codes. */
+int read_slice(const char* path, SliceRec& r) {
+    static const std::regex name_type{"^.*\\.slice"};
+    if (!path || !std::regex_match(path, name_type)) return 1;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return open_code(errno);
+    /* a blob longer than the bytes left is a short read, found before any allocation */
+    uint64_t fsize = 0;
+    if (std::fseek(fp, 0, SEEK_END) == 0) {
+        const long e = std::ftell(fp);
+        fsize = e > 0 ? static_cast<uint64_t>(e) : 0;
+    }
+    std::rewind(fp);
+    auto left = [&]() -> uint64_t {
+        const long at = std::ftell(fp);
+        return at >= 0 && static_cast<uint64_t>(at) <= fsize ? fsize - static_cast<uint64_t>(at) : 0;
+    };
+    auto rd = [&](void* p, size_t n) { return std::fread(p, 1, n, fp) == n; };
+    int rc = 0;
+    if (!rd(&r.type, 1)) rc = 5;
+    else if (!(r.type & 0x01)) rc = 7;
+    else if (!rd(&r.timestamp, 4) || !rd(&r.index, 4) || !rd(&r.size, 8) || !rd(&r.qp, 1) || !rd(r.mv_cm, 64)) rc = 5;
+    if (!rc && !(r.type & 0x02)) {
+        uint64_t g = 0;
+        if (!rd(&g, 8)) rc = 5;
+        else if (!g) rc = 8;
+        else if (g > left()) rc = 5;
+        else {
+            r.geo.resize(g);
+            if (!rd(r.geo.data(), g)) rc = 5;
+            r.has_geo = !rc;
+        }
+    }
+    if (!rc && !(r.type & 0x04)) {
+        uint64_t c = 0;
+        if (!rd(&c, 8)) rc = 5;
+        else if (!c) rc = 8;
+        else if (c > left()) rc = 5;
+        else {
+            r.col.resize(c);
+            if (!rd(r.col.data(), c)) rc = 5;
+            r.has_col = !rc;
+        }
+    }
+    std::fclose(fp);
+    return rc;
+}
+
+template <class T>
+void put(std::vector<uint8_t>& o, const T& v) {
+    const auto* p = reinterpret_cast<const uint8_t*>(&v);
+    o.insert(o.end(), p, p + sizeof(T));
+}
+
+/* The .frame image ChangeSliceToFrame writes for these slices
+ * (slice_io.cc:238-310, common.h:378-412). */
+int frame_image(std::vector<SliceRec>& sl, std::vector<uint8_t>& img) {
+    if (sl.empty()) return 9;
+    std::stable_sort(sl.begin(), sl.end(), [](const SliceRec& a, const SliceRec& b) { return a.index < b.index; });
+    const uint32_t S = static_cast<uint32_t>(sl.size());
+    img.clear();
+    put(img, sl.front().timestamp);
+    put(img, S);
+    for (auto& r : sl) put(img, r.index);
+    for (auto& r : sl) put(img, r.type);
+    for (auto& r : sl) put(img, static_cast<uint32_t>(r.size));
+    for (auto& r : sl) put(img, r.qp);
+    for (auto& r : sl)
+        for (int row = 0; row < 4; ++row)
+            for (int col = 0; col < 4; ++col) put(img, r.mv_cm[4 * col + row]);
+    for (auto& r : sl) put(img, static_cast<uint32_t>(r.has_geo ? r.geo.size() : 0));
+    for (auto& r : sl)
+        if (r.has_geo) img.insert(img.end(), r.geo.begin(), r.geo.end());
+    for (auto& r : sl) put(img, static_cast<uint32_t>(r.has_col ? r.col.size() : 0));
+    for (auto& r : sl)
+        if (r.has_col) img.insert(img.end(), r.col.begin(), r.col.end());
+    return 0;
+}
+
+int read_slices(const char* const* paths, int n, std::vector<SliceRec>& sl) {
+    if (n <= 0 || !paths) return 9;
+    sl.assign(static_cast<size_t>(n), SliceRec{});
+    for (int i = 0; i < n; ++i)
+        if (int rc = read_slice(paths[i], sl[i])) return rc;
+    return 0;
+}
+
+int write_file(const char* path, const std::vector<uint8_t>& img) {
+    static const std::regex out_type{"^.*\\.frame"};
+    if (!path || !std::regex_match(path, out_type)) return 1;
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return open_code(errno);
+    const bool ok = img.empty() || std::fwrite(img.data(), 1, img.size(), fp) == img.size();
+    return (std::fclose(fp) == 0 && ok) ? 0 : 10;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t pvvc_save_slice(const char* path, const pvvc_slice_desc* s) {
+    static const std::regex name_type{"^.*\\.slice"};
+    if (!path || !s || !std::regex_match(path, name_type)) return -1;
+    if (!(s->type & 0x01)) return -7;
+    const bool geo = !(s->type & 0x02), col = !(s->type & 0x04);
+    if ((geo && (!s->geometry || !s->geometry_size)) || (col && (!s->color || !s->color_size))) return -8;
+    std::vector<uint8_t> o;
+    put(o, s->type);
+    put(o, s->timestamp);
+    put(o, s->index);
+    put(o, s->size);
+    put(o, s->qp);
+    for (int col_i = 0; col_i < 4; ++col_i)
+        for (int row = 0; row < 4; ++row) put(o, s->mv[4 * row + col_i]); /* stored column-major */
+    if (geo) {
+        put(o, s->geometry_size);
+        o.insert(o.end(), s->geometry, s->geometry + s->geometry_size);
+    }
+    if (col) {
+        put(o, s->color_size);
+        o.insert(o.end(), s->color, s->color + s->color_size);
+    }
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return -open_code(errno);
+    const bool ok = std::fwrite(o.data(), 1, o.size(), fp) == o.size();
+    if (std::fclose(fp) != 0 || !ok) return -10;
+    return static_cast<int64_t>(o.size());
+}
+
+int pvvc_load_slices(const char* const* paths, int n, pvvc_frame** out) {
+    if (!out) return PVVC_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::vector<SliceRec> sl;
+    std::vector<uint8_t> img;
+    int rc = read_slices(paths, n, sl);
+    if (!rc) rc = frame_image(sl, img);
+    if (rc) return rc;
+    return pvvc_parse_frame(img.data(), img.size(), out);
+}
+
+int pvvc_slices_to_frame_file(const char* const* paths, int n, const char* frame_path) {
+    std::vector<SliceRec> sl;
+    std::vector<uint8_t> img;
+    int rc = read_slices(paths, n, sl);
+    if (!rc) rc = frame_image(sl, img);
+    return rc ? rc : write_file(frame_path, img);
+}
+
+int pvvc_change_slice_to_frame(const char* input_dir, const char* frame_path) {
+    static const std::regex out_type{"^.*\\.frame"};
+    if (!frame_path || !std::regex_match(frame_path, out_type)) return 1;
+    std::error_code ec;
+    if (!input_dir || !std::filesystem::is_directory(input_dir, ec)) return 4;
+    static const std::regex slice_name{"^.*_time_[0-9]+_slice_[0-9]+\\.slice$"};
+    std::vector<std::string> names;
+    for (const auto& e : std::filesystem::directory_iterator(input_dir, ec)) {
+        const std::string nm = e.path().string();
+        if (std::regex_match(nm, slice_name)) names.push_back(nm);
+    }
+    if (ec) return 4;
+    if (names.empty()) return 2;
+    std::sort(names.begin(), names.end()); /* a fixed order before the sort by index */
+    std::vector<const char*> p;
+    for (auto& nm : names) p.push_back(nm.c_str());
+    return pvvc_slices_to_frame_file(p.data(), static_cast<int>(p.size()), frame_path);
 }
 
 }  // extern "C"

@@ -19,7 +19,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 from . import _native
-from ._native import DropinFrame, Points, SynthConfig, Tuning, enc_lib, hip_lib
+from ._native import DropinFrame, Points, SliceDesc, SynthConfig, Tuning, enc_lib, hip_lib
 
 __all__ = [
     "PvvcError", "StagedFrame", "launch_dropin", "synchronize", "last_kernel_ms", "last_launch_times",
@@ -82,6 +82,16 @@ class Frame:
             raise PvvcError(rc, "pvvc_parse_frame")
         return cls(h.value)
 
+    @classmethod
+    def from_slices(cls, paths) -> "Frame":
+        """The frame of .slice files (pvvc_load_slices: LoadSlice + ChangeSliceToFrame)."""
+        h = C.c_void_p()
+        arr = (C.c_char_p * max(1, len(paths)))(*[str(p).encode() for p in paths])
+        rc = hip_lib().pvvc_load_slices(arr, len(paths), C.byref(h))
+        if rc != 0:
+            raise PvvcError(rc, "pvvc_load_slices")
+        return cls(h.value)
+
     def close(self) -> None:
         if self._h and self._h.value:
             hip_lib().pvvc_frame_free(self._h)
@@ -134,6 +144,43 @@ class Frame:
 # ---------------------------------------------------------------------------
 # Device memory
 # ---------------------------------------------------------------------------
+
+SLICE_CODES = {1: "wrong file name", 2: "no such file", 3: "permission denied", 4: "open error / not a directory",
+               5: "short read", 6: "zstd", 7: "slice type without the valid bit", 8: "empty blob", 9: "no slices",
+               10: "write error"}
+
+
+def save_slice(path, type_: int, timestamp: int, index: int, size: int, qp: int, mv, geometry: bytes | None = None,
+               color: bytes | None = None) -> int:
+    """pvvc_save_slice (io::SaveSlice): one .slice file; mv is 4x4 row-major.  Returns the bytes written."""
+    d = SliceDesc()
+    d.type, d.timestamp, d.index, d.size, d.qp = type_, timestamp, index, size, qp
+    d.mv[:] = [float(x) for x in np.asarray(mv, np.float32).reshape(16)]
+    keep = []
+    for name, blob in (("geometry", geometry), ("color", color)):
+        if blob:
+            b = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
+            keep.append(b)
+            setattr(d, name, C.cast(b, C.c_void_p).value)
+            setattr(d, name + "_size", len(blob))
+    n = hip_lib().pvvc_save_slice(str(path).encode(), C.byref(d))
+    if n < 0:
+        raise PvvcError(-n, f"pvvc_save_slice({path}): {SLICE_CODES.get(-n, '')}")
+    return int(n)
+
+
+def slices_to_frame_file(paths, frame_path) -> None:
+    """pvvc_slices_to_frame_file: the .frame file ChangeSliceToFrame writes for these slices."""
+    arr = (C.c_char_p * max(1, len(paths)))(*[str(p).encode() for p in paths])
+    _check(hip_lib().pvvc_slices_to_frame_file(arr, len(paths), str(frame_path).encode()), "pvvc_slices_to_frame_file")
+
+
+def change_slice_to_frame(input_dir, frame_path) -> None:
+    """pvvc_change_slice_to_frame (io::ChangeSliceToFrame over a directory)."""
+    _check(hip_lib().pvvc_change_slice_to_frame(str(input_dir).encode(), str(frame_path).encode()),
+           "pvvc_change_slice_to_frame")
+
+
 class DeviceBuffer:
     """A plain hipMalloc allocation."""
 

@@ -31,6 +31,13 @@ class DropinFrame(C.Structure):
     ]
 
 
+class SliceDesc(C.Structure):
+    """pvvc_slice_desc: one .slice file's fields (mv row-major)."""
+    _fields_ = [("type", C.c_uint8), ("timestamp", C.c_int32), ("index", C.c_int32), ("size", C.c_uint64),
+                ("qp", C.c_uint8), ("mv", C.c_float * 16), ("geometry", c_vp), ("geometry_size", C.c_uint64),
+                ("color", c_vp), ("color_size", C.c_uint64)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("waves_per_patch", C.c_int), ("lds_bytes", C.c_int), ("wgs_per_cu", C.c_int),
                 ("queue_chunk", C.c_int), ("priority_items", C.c_int), ("lane_max_points", C.c_int)]
@@ -94,6 +101,10 @@ def hip_lib() -> C.CDLL:
     L = C.CDLL(str(path))
     _sig(L, "pvvc_load_frame", C.c_int, C.c_char_p, C.POINTER(c_vp))
     _sig(L, "pvvc_parse_frame", C.c_int, c_vp, C.c_size_t, C.POINTER(c_vp))
+    _sig(L, "pvvc_save_slice", C.c_int64, C.c_char_p, C.POINTER(SliceDesc))
+    _sig(L, "pvvc_load_slices", C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(c_vp))
+    _sig(L, "pvvc_slices_to_frame_file", C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_char_p)
+    _sig(L, "pvvc_change_slice_to_frame", C.c_int, C.c_char_p, C.c_char_p)
     _sig(L, "pvvc_frame_free", None, c_vp)
     _sig(L, "pvvc_set_host_threads", C.c_int, C.c_int)
     _sig(L, "pvvc_frame_timestamp", C.c_int, c_vp)

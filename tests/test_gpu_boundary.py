@@ -9,7 +9,8 @@
 * two host threads, each with its own state, decoder and stream, decode two
   different streams concurrently, both bit-exact against the oracle;
 * a predictive slice that carries a geometry blob decodes on its patch's
-  intra tree, as the reference does (invert_raht_octree.cc:71-78).
+  intra tree, as the reference does (invert_raht_octree.cc:71-78);
+* a GoP read from .slice files decodes like its .frame files.
 """
 import ctypes as C
 import sys
@@ -218,3 +219,32 @@ def test_predictive_slice_geometry_blob_ignored(pvvc, oracle, c0_stream, tmp_pat
         for k, ref in enumerate(refs):
             o, n = b.frame_offset[k], b.frame_points[k]
             np.testing.assert_array_equal(rgb[o:o + n], ref["rgb"])
+
+
+def test_gop_from_slice_files_matches_oracle(pvvc, oracle, c0_stream, tmp_path):
+    """A c0 GoP written as .slice files (one per patch and frame), read back
+    with pvvc_load_slices (LoadSlice + ChangeSliceToFrame, the reference
+    decoder harness's input, src/codec/pvvc_decompression.cc:34-84), decodes
+    in one persistent launch bit-exactly like the oracle decodes the .frame
+    files."""
+    from test_slice_io import pack_slice, split_frame
+    refs = _oracle_refs(oracle, tmp_path, "c0s", c0_stream)
+    frames = []
+    for k, image in enumerate(c0_stream):
+        ts, slices = split_frame(image)
+        paths = []
+        for s in slices:
+            p = tmp_path / f"f{k}_time_{ts:04d}_slice_{s['index']:04d}.slice"
+            p.write_bytes(pack_slice(s["type"], ts, s["index"], s["size"], s["qp"], s["mv"], s["geo"] or None,
+                                     s["col"]))
+            paths.append(p)
+        frames.append(pvvc.Frame.from_slices(paths[::-1]))
+    _say("GoP from", sum(f.slice_cnt for f in frames), "slice files")
+    dec = pvvc.Decoder(*_dims(frames))
+    b = dec.batch(frames)
+    xyz, rgb = dec.decode_host(b, pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+    assert (b.status() == 0).all()
+    for k, ref in enumerate(refs):
+        o, n = b.frame_offset[k], b.frame_points[k]
+        np.testing.assert_array_equal(xyz[o:o + n], ref["xyz"])
+        np.testing.assert_array_equal(rgb[o:o + n], ref["rgb"])
