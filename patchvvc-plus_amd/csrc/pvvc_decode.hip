@@ -148,7 +148,10 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  *    then the Golomb-Rice coded symbol + 1 and the adaptation of :240-251;
  *  - the raw Golomb value u is written by v_writelane into lane
  *    (nrel - lim) mod 64 of `acc`, selected by the low bits of `neg`
- *    (Unsign2Sign is applied when the chunk is stored);
+ *    (Unsign2Sign is applied when the chunk is stored); in the k = 0 loop
+ *    the write sits between the zero test and its branch, where the branch
+ *    waits for the scalar pipeline anyway (a zero symbol writes 0 into a
+ *    lane that holds 0);
  *  - the window takes 32 more bits when fewer than 33 remain.  The code's
  *    length is subtracted from nbx as soon as it is known: no borrow (the
  *    common case) means the code lies in the window's first 32 bits and no
@@ -222,10 +225,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
         "12:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -244,10 +247,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
         "14:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -266,10 +269,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
         "52:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -288,10 +291,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
         "56:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -306,10 +309,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
@@ -331,10 +334,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
@@ -356,10 +359,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
@@ -381,10 +384,10 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b32 s89, s84, s85\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_or_b32 s89, s89, s82\n\t"
+        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
