@@ -338,7 +338,7 @@ int pvvc_decode_batch_noalloc(pvvc_decoder* dec, pvvc_batch* batch, int output, 
 
 /* Autotuner: picks waves per slice (1/2/4), the LDS budget per slice
  * (4-64 KiB) and, for the persistent mode, workgroups per CU, the queue
- * chunk and the lane decoder's slice size limit (off, 64, 256, 1000 points)
+ * chunk and the lane decoder's slice size limit (off, 256, 1000, 2000, 4000 points)
  * by coordinate descent, timing `reps` decodes of `batch` per
  * candidate (HIP events; median).  Sets the decoder's tuning to the best
  * and returns it with its time; the outputs hold that configuration's

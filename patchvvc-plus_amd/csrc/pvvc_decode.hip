@@ -3672,7 +3672,7 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
     static const int kLds[] = {4096, 8192, 16384, 24576, 32768, 49152, 65536};
     static const int kWgs[] = {2, 4, 8, 12, 16, 20, 24, 32};
     static const int kChunk[] = {1, 2, 4, 8};
-    static const int kLane[] = {0, 64, 256, 1000}; /* lane_max_points: 0 = the scalar decoder only */
+    static const int kLane[] = {0, 256, 1000, 2000, 4000}; /* lane_max_points: 0 = the scalar decoder only */
     const bool persistent = mode == PVVC_MODE_PERSISTENT;
     std::map<std::tuple<int, int, int, int, int>, double> seen;
     auto measure = [&](const pvvc_tuning& t, double& ms) -> int {
@@ -3706,7 +3706,7 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
         for (int dim = 0; dim < 5; ++dim) {
             if (!persistent && dim >= 2) break; /* grid shape is the slice count there */
             const int* vals = dim == 0 ? kWaves : dim == 1 ? kLds : dim == 2 ? kWgs : dim == 3 ? kChunk : kLane;
-            const int nv = dim == 0 ? 3 : dim == 1 ? 7 : dim == 2 ? 8 : 4;
+            const int nv = dim == 0 ? 3 : dim == 1 ? 7 : dim == 2 ? 8 : dim == 3 ? 4 : 5;
             for (int k = 0; k < nv; ++k) {
                 pvvc_tuning t = cur;
                 int* field = dim == 0   ? &t.waves_per_patch

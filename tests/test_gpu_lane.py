@@ -8,8 +8,9 @@ stream while the scalar decoder takes the long slices.
   the slices that decode equal those of the scalar decoder;
 * a c1 GoP (10-bit, ~816k points/frame, GoP 8, qp 4) at several thresholds
   against the oracle, a c2 GoP (16 frames, 15 of them predictive and deferred
-  to the finish pass) against the scalar decoder, and a batch of two GoPs
-  (two segments), all in one decoder whose lane buffers grow and are reused.
+  to the finish pass) and 8 frames of c4 (qp 1, slices up to 4 000 points on
+  the lanes) against the scalar decoder, and a batch of two GoPs (two
+  segments), all in one decoder whose lane buffers grow and are reused.
 """
 import sys
 
@@ -115,6 +116,16 @@ def test_lane_c1_c2_and_two_segments(pvvc, oracle, tmp_path):
         assert (s1 == 0).all() and n_launch == 1
         np.testing.assert_array_equal(x1, x0)
         np.testing.assert_array_equal(r1, r0)
+    c4 = [pvvc.Frame.parse(b) for b in pvvc.synth_stream("c4", n_frames=8, seed=6, threads=16)]
+    d4 = pvvc.Decoder(*_dims(c4))
+    x0, r0, s0, _, _ = _decode(pvvc, d4, c4, -1)
+    assert (s0 == 0).all()
+    x1, r1, s1, _, n_launch = _decode(pvvc, d4, c4, 4000)
+    _say("  c4 (8 frames, qp 1) lane_max_points 4000: equal to the scalar decode")
+    assert (s1 == 0).all() and n_launch == 1
+    np.testing.assert_array_equal(x1, x0)
+    np.testing.assert_array_equal(r1, r0)
+    d4.close()
     two = f1 + f1  # the second intra frame starts a second segment
     xa, ra, sa, _, na = _decode(pvvc, dec, two, -1)
     xb, rb, sb, _, nb = _decode(pvvc, dec, two, 1000)

@@ -104,7 +104,7 @@ def test_autotune_keeps_output(pvvc, c0_files, c0_oracle):
     assert best == dec.tuning()
     assert best["waves_per_patch"] in (1, 2, 4)
     assert 4096 <= best["lds_bytes"] <= 65536
-    assert best["lane_max_points"] in (0, 64, 256, 1000)
+    assert best["lane_max_points"] in (0, 256, 1000, 2000, 4000)
     assert ms > 0
     n = b.total_points
     xyz = out.xyz.read(np.float32, 3 * n).reshape(n, 3)
