@@ -1622,7 +1622,8 @@ __global__ __launch_bounds__(64) void lane_rlgr_kernel(const ItemDev* __restrict
     const uintptr_t a = reinterpret_cast<uintptr_t>(src);
     const uint32_t skip = 8u * static_cast<uint32_t>(a & 3u);
     const uint32_t end = skip + 8u * it->col_size;
-    const LaneRing r{reinterpret_cast<const uint32_t*>(a & ~uintptr_t(3)), end ? (end - 1u) / 32u : 0u, 0u, ring + lane};
+    const LaneRing r{reinterpret_cast<const uint32_t*>(a & ~uintptr_t(3)), end, end ? (end - 1u) / 32u : 0u, 0u,
+                     ring + lane};
     lane_status[i] = rlgr_decode_lane_from(r, skip, it->col_size, 3u * it->n, it->qp, coef_all + 3ull * it->stage_off);
 }
 

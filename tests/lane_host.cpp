@@ -14,7 +14,7 @@ extern "C" int lane_decode_ring(const uint8_t* src, uint32_t nbytes, uint32_t to
     const uintptr_t a = reinterpret_cast<uintptr_t>(src);
     const uint32_t skip = 8u * static_cast<uint32_t>(a & 3u);
     const uint32_t end = skip + 8u * nbytes;
-    const pvvc::LaneRing r{reinterpret_cast<const uint32_t*>(a & ~uintptr_t(3)), end ? (end - 1u) / 32u : 0u, 0u,
-                           ring.data() + 17};
+    const pvvc::LaneRing r{reinterpret_cast<const uint32_t*>(a & ~uintptr_t(3)), end, end ? (end - 1u) / 32u : 0u,
+                           0u, ring.data() + 17};
     return pvvc::rlgr_decode_lane_from(r, skip, nbytes, total, qp, coef);
 }
