@@ -3139,8 +3139,10 @@ bool grow_dev(T*& p, uint64_t& cap, uint64_t need, hipStream_t s) {
  * decodes the tail's RLGR one slice per lane, then a persistent launch with
  * those coefficients does the rest of their decode (own queue counter and
  * workspaces).  On s, meanwhile, the scalar persistent launch decodes the
- * head [0, ns).  The lane kernel is enqueued first so that its wavefronts
- * (few registers, no LDS) are resident beside the scalar launch's.  The
+ * head [0, ns).  The lane kernel is enqueued first, but the command
+ * processor may dispatch the scalar launch first, whose workgroups then fill
+ * every CU (VGPRs) and hold the lane kernel's 33 KiB workgroups off until they
+ * drain: the span varies from launch to launch (c2: 4.7 to 5.7 ms).  The
  * streams join before the finish pass of the deferred predictive slices. */
 int launch_segment_lanes(pvvc_decoder* d, pvvc_batch* b, const BatchDev& bd, const Segment& sg, uint32_t ns,
                          const Sink& sink, const pvvc_tuning& t, const LaunchRes& res, uint32_t launch_id,
