@@ -1757,134 +1757,202 @@ __device__ __forceinline__ bool quantise(float c, float qp, int32_t& q) {
     return true;
 }
 
-/* MSB-first bit writer of one wavefront: whole 32-bit words are gathered in
- * lane (word mod 64) of `stage` and stored 256 bytes at a time. */
-struct BitWriterW {
-    uint64_t buf;
-    uint32_t cnt;     /* pending bits, < 32 between calls */
-    uint32_t words;   /* words emitted */
-    uint32_t stage;
-    uint8_t* out;
-    uint32_t cap_words;
-    bool over;
-};
+/* RLGR encoder (entropy_codec.cc:22-159), one wavefront per stream, 64
+ * symbols per step:
+ *  1. lane j loads symbol j of the step and maps it to u (Sign2Unsign);
+ *  2. a wave-uniform pass walks the step with the adaptive state (k_P, k_RP,
+ *     m) and leaves in each lane what its symbol writes: a Golomb-Rice code
+ *     with parameter k_R (k = 0, or the end of a zero run, which first writes
+ *     '0' and m in k bits), or one '1' bit (the zero at which m reaches 2^k),
+ *     or nothing (any other zero of a run).  Zero runs are crossed by whole
+ *     runs, found in the ballot of nonzero symbols, one step per '1' bit;
+ *  3. the lanes' code lengths are scanned into bit offsets, and each lane ORs
+ *     its bits into an LDS window of the output (MSB first);
+ *  4. the window's whole words are stored (byte-swapped, so the stream is
+ *     big-endian as the reference writes it) and its open word carried.
+ * Byte-identical to the reference writer, including its end rule (a stream
+ * whose last symbol leaves k > 0 and u = 0 gets '0' and m in k_P / L bits),
+ * the 32-ones escape and its k > 32 failure. */
+constexpr uint32_t kEncWindow = 256; /* words: one step writes at most 64 x 97 bits, plus the open word */
 
-__device__ __forceinline__ void bw_store_chunk(BitWriterW& w, uint32_t first_word, uint32_t count) {
-    const uint32_t lane = lane_id();
-    const uint32_t wi = first_word + lane;
-    if (lane < count) {
-        if (wi < w.cap_words) reinterpret_cast<uint32_t*>(w.out)[wi] = w.stage;
+/* ORs `len` <= 33 bits of `v` (v < 2^len) at bit `pos` of win (MSB first). */
+__device__ __forceinline__ void enc_or_bits(uint32_t* win, uint32_t pos, uint64_t v, uint32_t len) {
+    if (len == 0 || v == 0) return;
+    const uint64_t x = (v << (64u - len)) >> (pos & 31u);
+    const uint32_t w = pos >> 5;
+    const uint32_t hi = static_cast<uint32_t>(x >> 32), lo = static_cast<uint32_t>(x);
+    if (hi) atomicOr(win + w, hi);
+    if (lo) atomicOr(win + w + 1, lo);
+}
+
+/* Exclusive wave scan of v < 128 by bit planes; *total gets the sum. */
+__device__ __forceinline__ uint32_t wave_scan_128(uint32_t v, uint32_t* total) {
+    uint32_t r = 0, t = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) {
+        const uint64_t mb = __ballot((v >> b) & 1u);
+        r += prefix_lt(mb) << b;
+        t += static_cast<uint32_t>(__popcll(mb)) << b;
     }
-    if (first_word + count > w.cap_words) w.over = true;
+    *total = t;
+    return r;
 }
 
-__device__ __forceinline__ void bw_word(BitWriterW& w, uint32_t word) {
-    const uint32_t lane = lane_id();
-    w.stage = (lane == (w.words & 63u)) ? __builtin_bswap32(word) : w.stage;
-    ++w.words;
-    if ((w.words & 63u) == 0) bw_store_chunk(w, w.words - 64u, 64u);
+/* Stores window words [0, count) at output word w0 (those below cap_words). */
+__device__ __forceinline__ void enc_store(const uint32_t* win, uint32_t count, uint8_t* out, uint32_t w0,
+                                          uint32_t cap_words) {
+    uint32_t* o = reinterpret_cast<uint32_t*>(out);
+    for (uint32_t i = lane_id(); i < count; i += 64u)
+        if (w0 + i < cap_words) o[w0 + i] = __builtin_bswap32(win[i]);
 }
 
-/* Write(x, bits) (entropy_codec.cc:22-36) for bits <= 32, x < 2^bits. */
-__device__ __forceinline__ void bw_put(BitWriterW& w, uint64_t x, uint32_t bits) {
-    if (bits == 0) return;
-    w.buf = (w.buf << bits) + x;
-    w.cnt += bits;
-    if (w.cnt >= 32) {
-        w.cnt -= 32;
-        bw_word(w, static_cast<uint32_t>(w.buf >> w.cnt));
-    }
+/* k_RP after a Golomb-Rice code of x with parameter kR (entropy_codec.cc:
+ * 92-100): p = x >> kR; p ? min(k_RP + p - 1, 128) : max(k_RP - 2, 0).  32-bit
+ * scalar arithmetic (p is capped at 129, which saturates the same way). */
+__device__ __forceinline__ uint32_t enc_adapt_krp(uint32_t kRP, uint32_t x, uint32_t kR) {
+    const uint32_t p = static_cast<uint32_t>(static_cast<uint64_t>(x) >> kR);
+    const int32_t up = static_cast<int32_t>(kRP + (p < 129u ? p : 129u)) - 1;
+    const int32_t down = static_cast<int32_t>(kRP) - 2;
+    return p ? static_cast<uint32_t>(up < 128 ? up : 128) : static_cast<uint32_t>(down > 0 ? down : 0);
 }
 
-/* GRWrite (entropy_codec.cc:38-63) */
-__device__ __forceinline__ void bw_gr(BitWriterW& w, uint64_t x, uint32_t k) {
-    const uint64_t p = x >> k;
-    if (p < 32) {
-        bw_put(w, (uint64_t{1} << (p + 1)) - 2, static_cast<uint32_t>(p) + 1);
-        bw_put(w, k >= 64 ? x : (x & ((uint64_t{1} << k) - 1)), k);
-    } else {
-        bw_put(w, 0xffffffffull, 32);
-        bw_put(w, x & 0xffffffffull, 32);
-    }
-}
+/* Lane descriptor bits of one step. */
+constexpr uint32_t kEncGr = 1u << 31;    /* a Golomb-Rice code of u (minus 1 at a run end), k_R in bits 0-5 */
+constexpr uint32_t kEncRunEnd = 1u << 30; /* '0' and m in k bits first, k in bits 8-13 */
+constexpr uint32_t kEncOne = 1u << 29;   /* a single '1' */
 
-/* RLGR encode of sym[0, total) (entropy_codec.cc:65-159); one wavefront,
- * wave-uniform control.  Returns the byte length, or -1 past out_cap. */
+/* win: kEncWindow words of LDS, owned by this wavefront.  Returns the byte
+ * length, or -1 past out_cap or on k > 32. */
 __device__ __forceinline__ int64_t rlgr_encode_wave(const int32_t* __restrict__ sym, uint32_t total, uint8_t* out,
-                                                    uint32_t out_cap) {
+                                                    uint32_t out_cap, uint32_t* win) {
     const uint32_t lane = lane_id();
-    BitWriterW w{0, 0, 0, 0, out, out_cap / 4u, false};
-    uint64_t u = 0, kP = 0, kRP = 8, m = 0, k = 0, kR = 0;
+    const uint32_t cap_words = out_cap / 4u;
+    for (uint32_t i = lane; i < kEncWindow; i += 64u) win[i] = 0u;
+    __syncthreads();
+    uint32_t kP = 0, kRP = 8, m = 0;
+    bool last_run = false, last_u0 = false; /* the end rule's k != 0 and u == 0 */
+    uint64_t bitpos = 0;                      /* bits written before this step */
     for (uint32_t base = 0; base < total; base += 64u) {
-        const int32_t chunk = (base + lane < total) ? sym[base + lane] : 0;
         const uint32_t lim = total - base < 64u ? total - base : 64u;
-        for (uint32_t j = 0; j < lim; ++j) {
-            const int64_t d = __builtin_amdgcn_readlane(chunk, static_cast<int>(j));
-            u = d < 0 ? (static_cast<uint64_t>(-d) << 1) - 1 : static_cast<uint64_t>(d) << 1;
-            k = kP >> 2;
-            kR = kRP >> 2;
-            if (k > 32) { /* needs > 2^32 zeros in a run; the decoder rejects k > 30 */
-                w.over = true;
-                return -1;
-            }
-            if (k) {
-                if (u) {
-                    --u;
-                    bw_put(w, 0, 1);
-                    bw_put(w, m, static_cast<uint32_t>(k));
-                    bw_gr(w, u, static_cast<uint32_t>(kR));
-                    const uint64_t p = u >> kR;
-                    if (p) {
-                        kRP += p - 1;
-                        kRP = kRP > 128 ? 128 : kRP;
-                    } else {
-                        kRP = kRP < 2 ? 0 : kRP - 2;
-                    }
-                    kP = kP < 1 ? 0 : kP - 1;
-                    m = 0;
-                } else if (++m == (uint64_t{1} << k)) {
-                    bw_put(w, 1, 1);
-                    kP += 2;
-                    m = 0;
-                }
-            } else {
-                bw_gr(w, u, static_cast<uint32_t>(kR));
-                const uint64_t p = u >> kR;
-                if (p) {
-                    kRP = kRP + p - 1 > 128 ? 128 : kRP + p - 1;
-                } else {
-                    kRP = kRP < 2 ? 0 : kRP - 2;
-                }
-                kP = u ? (kP < 1 ? 0 : kP - 1) : kP + 3;
+        const int64_t d = base + lane < total ? sym[base + lane] : 0;
+        const uint32_t u = static_cast<uint32_t>(d < 0 ? (static_cast<uint64_t>(-d) << 1) - 1u
+                                                       : static_cast<uint64_t>(d) << 1);
+        const uint64_t nz = __ballot(u != 0u);
+        uint32_t desc = 0, mm = 0;
+        uint32_t j = 0;
+        while (j < lim) {
+            uint32_t k = kP >> 2;
+            if (k > 32u) return -1;
+            if (k == 0u) {
+                /* k = 0: every symbol is a Golomb-Rice code */
+                uint32_t uj = __builtin_amdgcn_readlane(u, j);
+                do {
+                    const uint32_t un = __builtin_amdgcn_readlane(u, j + 1u < lim ? j + 1u : j);
+                    const uint32_t kR = kRP >> 2;
+                    desc = lane == j ? (kEncGr | kR) : desc;
+                    kRP = enc_adapt_krp(kRP, uj, kR);
+                    kP = uj ? (kP > 1u ? kP : 1u) - 1u : kP + 3u;
+                    uj = un;
+                    ++j;
+                } while (j < lim && kP < 4u);
+                last_run = false;
+            } else if ((nz >> j) & 1u) {
+                const uint32_t uj = __builtin_amdgcn_readlane(u, j) - 1u;
+                const uint32_t kR = kRP >> 2;
+                desc = lane == j ? (kEncGr | kEncRunEnd | kR | (k << 8)) : desc;
+                mm = lane == j ? m : mm;
+                kRP = enc_adapt_krp(kRP, uj, kR);
+                kP = (kP > 1u ? kP : 1u) - 1u;
                 m = 0;
+                last_run = true;
+                last_u0 = uj == 0u;
+                ++j;
+            } else {
+                /* zeros [j, t): m counts them; each time it reaches 2^k the
+                 * zero there writes '1', k_P += 2 and m restarts */
+                const uint64_t rest = nz >> j;
+                const uint32_t t = rest ? j + static_cast<uint32_t>(__builtin_ctzll(rest)) : lim;
+                const uint32_t end = t < lim ? t : lim;
+                uint32_t z = end - j;
+                last_run = true;
+                last_u0 = true;
+                while (z) {
+                    if (k > 32u) return -1;
+                    const uint64_t need = (uint64_t{1} << k) - m;
+                    if (z < need) {
+                        m += z;
+                        j += z;
+                        z = 0;
+                    } else {
+                        const uint32_t at = j + static_cast<uint32_t>(need) - 1u;
+                        desc = lane == at ? kEncOne : desc;
+                        j += static_cast<uint32_t>(need);
+                        z -= static_cast<uint32_t>(need);
+                        m = 0;
+                        kP += 2u;
+                        k = kP >> 2;
+                    }
+                }
             }
         }
+        /* this lane's bits: [run-end prefix][GR prefix][GR suffix] or a '1' */
+        uint32_t l1 = 0, l2 = 0, l3 = 0;
+        uint64_t v2 = 0, v3 = 0;
+        if (desc & kEncOne) {
+            l2 = 1;
+            v2 = 1;
+        } else if (desc & kEncGr) {
+            const uint32_t kR = desc & 63u;
+            const uint64_t x = (desc & kEncRunEnd) ? u - 1u : u;
+            if (desc & kEncRunEnd) l1 = 1u + ((desc >> 8) & 63u);
+            const uint64_t q = x >> kR;
+            if (q < 32u) {
+                l2 = static_cast<uint32_t>(q) + 1u;
+                v2 = (uint64_t{1} << l2) - 2u;
+                l3 = kR;
+                v3 = kR >= 64u ? x : (x & ((uint64_t{1} << kR) - 1u));
+            } else {
+                l2 = 32;
+                v2 = 0xffffffffull;
+                l3 = 32;
+                v3 = x & 0xffffffffull;
+            }
+        }
+        uint32_t step_bits;
+        const uint32_t off = wave_scan_128(l1 + l2 + l3, &step_bits);
+        const uint32_t w0 = static_cast<uint32_t>(bitpos >> 5);
+        const uint32_t rel = static_cast<uint32_t>(bitpos & 31u) + off;
+        enc_or_bits(win, rel, mm, l1);
+        enc_or_bits(win, rel + l1, v2, l2);
+        enc_or_bits(win, rel + l1 + l2, v3, l3);
+        __syncthreads(); /* one wavefront: the ORs have landed */
+        bitpos += step_bits;
+        const uint32_t done = static_cast<uint32_t>(bitpos >> 5) - w0; /* whole words of the window */
+        if (done) {
+            enc_store(win, done, out, w0, cap_words);
+            const uint32_t open = win[done];
+            __syncthreads();
+            for (uint32_t i = lane; i <= done; i += 64u) win[i] = i == 0 ? open : 0u;
+            __syncthreads();
+        }
     }
-    if (total > 0 && k && !u) {
-        bw_put(w, 0, 1);
-        bw_put(w, m, static_cast<uint32_t>(kP >> 2));
+    /* the window's word 0 is output word bitpos >> 5 */
+    const uint32_t w_open = static_cast<uint32_t>(bitpos >> 5);
+    if (total > 0 && last_run && last_u0) {
+        const uint32_t kb = kP >> 2;
+        if (lane == 0) enc_or_bits(win, static_cast<uint32_t>(bitpos & 31u) + 1u, m, kb); /* m < 2^kb */
+        bitpos += 1u + kb;
     }
-    const uint32_t r = w.cnt % 8u;
-    if (r) bw_put(w, 0, 8 - r);
-    const uint64_t bytes = 4ull * w.words + w.cnt / 8u;
-    /* words of the open chunk not yet stored (a full chunk was stored by bw_word) */
-    bool pending = (w.words & 63u) != 0;
-    if (w.cnt) { /* the last partial word, left aligned; it may complete the chunk */
-        const uint32_t word = static_cast<uint32_t>(w.buf << (32u - w.cnt));
-        w.stage = (lane == (w.words & 63u)) ? __builtin_bswap32(word) : w.stage;
-        ++w.words;
-        pending = true;
-    }
-    if (pending) {
-        const uint32_t first = (w.words - 1u) & ~63u;
-        bw_store_chunk(w, first, w.words - first);
-    }
-    return w.over ? -1 : static_cast<int64_t>(bytes);
+    bitpos = (bitpos + 7u) & ~uint64_t(7);
+    __syncthreads();
+    const uint64_t words = (bitpos + 31u) >> 5;
+    enc_store(win, static_cast<uint32_t>(words - w_open), out, w_open, cap_words);
+    return words > cap_words ? -1 : static_cast<int64_t>(bitpos >> 3);
 }
 
 /* Encodes one colour stream with one wavefront.  Returns the byte length or
  * -(status). */
-__device__ __forceinline__ int64_t encode_job(const EncJobDev& jb, Arena ar, LevelMeta& meta) {
+__device__ __forceinline__ int64_t encode_job(const EncJobDev& jb, Arena ar, LevelMeta& meta, uint32_t* win) {
     const uint32_t tid = threadIdx.x;
     const uint32_t n = jb.n;
     if (!jb.geo || jb.geo_size < 25 || n == 0 || !jb.yuv || !jb.out) return -PVVC_ERR_BAD_SLICE;
@@ -1981,7 +2049,7 @@ __device__ __forceinline__ int64_t encode_job(const EncJobDev& jb, Arena ar, Lev
     }
     __syncthreads();
     if (__ballot(!range_ok)) return -PVVC_ERR_SYMBOL_RANGE;
-    const int64_t bytes = rlgr_encode_wave(sym, 3u * n, jb.out, jb.out_cap);
+    const int64_t bytes = rlgr_encode_wave(sym, 3u * n, jb.out, jb.out_cap, win);
     return bytes < 0 ? -PVVC_ERR_CAPACITY : bytes;
 }
 
@@ -2007,6 +2075,7 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncJobDev* __restrict_
                                                     uint32_t lds_bytes, int64_t* __restrict__ result,
                                                     uint32_t wave1) {
     __shared__ LevelMeta meta;
+    __shared__ uint32_t win[kEncWindow];
     const uint32_t q = blockIdx.x;
     if (q >= count) return;
     const uint32_t rest = count > wave1 ? count - wave1 : 0u;
@@ -2015,7 +2084,7 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncJobDev* __restrict_
     else if (q < wave1 + 2 * (rest / 3)) __builtin_amdgcn_s_setprio(1);
     const EncJobDev jb = jobs[q];
     Arena ar{g_smem, lds_bytes, 0, jb.ws, jb.ws_cap, 0};
-    const int64_t r = encode_job(jb, ar, meta);
+    const int64_t r = encode_job(jb, ar, meta, win);
     if (threadIdx.x == 0) result[jb.idx] = r;
 }
 
