@@ -136,22 +136,31 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
 
 /* Common-case RLGR symbols as a tight loop on the scalar unit
  * (entropy_codec.cc:207-283).  State in SGPRs: the 64-bit window `win`
- * (left aligned, zero below its nb valid bits) with nbx = nb - 33, the next
- * dword `wpos` of the chunk held one dword per lane in `cur`, k_RP, k_P and
+ * (left aligned, zero below its nb valid bits; s[94:95] inside the loop, so
+ * that its top dword s95 can be named) with nbx = nb - 33, the next dword
+ * `wpos` of the chunk held one dword per lane in `cur`, k_RP, k_P and
  * neg = nrel - lim (mod 2^32; its carry out marks the end of the 64-symbol
  * chunk).  Inside the loop neg lives in M0, the lane select v_writelane
  * takes beside a scalar data operand.  Per symbol:
- *  - k = 0: Golomb-Rice code (unary prefix by one leading-ones count,
- *    remainder by shifts), k_P / k_RP adaptation (:255-276);
+ *  - k = 0: Golomb-Rice code (unary prefix by one leading-ones count, code
+ *    length = prefix + 1 + k_R), k_P / k_RP adaptation (:255-276);
  *  - k > 0: up to 6 one-bits of the run prefix (each adds 2^k, k_P += 2),
  *    the run length in k bits, the zero run (only advances the symbol index),
  *    then the Golomb-Rice coded symbol + 1 and the adaptation of :240-251;
- *  - the raw Golomb value u is written by v_writelane into lane
- *    (nrel - lim) mod 64 of `acc`, selected by the low bits of `neg`
- *    (Unsign2Sign is applied when the chunk is stored); in the k = 0 loop
- *    the write sits between the zero test and its branch, where the branch
- *    waits for the scalar pipeline anyway (a zero symbol writes 0 into a
- *    lane that holds 0);
+ *  - the scalar unit only finds where each code ends and adapts the state:
+ *    it does not assemble the symbol's value.  v_writelane into lane
+ *    (nrel - lim) mod 64 (selected by the low bits of `neg`) keeps what the
+ *    value is made from: aT = the 32 window bits that start with the
+ *    Golomb-Rice code, aK = k_R, and aF = 1 after a zero run (that symbol is
+ *    stored as u + 1).  When the chunk is stored, every lane decodes its
+ *    own value at once in vector code (lane_symbol: leading ones, a bit-field
+ *    extract), then Unsign2Sign and the dequantisation.  The writes are
+ *    vector instructions, which issue beside the scalar ones; they replace
+ *    five scalar instructions per symbol (two shifts, the OR of quotient and
+ *    remainder, the remainder's shift count, the + 1), at the price of one
+ *    compare for the p != 0 carry of the k_RP step.  A k = 0 symbol is 0
+ *    when the top k_R + 1 bits of the window are all zero; that test (one
+ *    shift) chooses the zero block;
  *  - the window takes 32 more bits when fewer than 33 remain.  The code's
  *    length is subtracted from nbx as soon as it is known: no borrow (the
  *    common case) means the code lies in the window's first 32 bits and no
@@ -165,11 +174,11 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * single slice's decode (the drop-in launch) is bound by that latency.  A
  * non-zero k = 0 symbol lowers k_P, so it stays in the k = 0 loop without a
  * mode test; a zero symbol (u = 0) leaves the line for the one zero block
- * (17) to raise k_P by 3, lower k_RP and test the mode, and its accumulator
- * lane already holds 0.  The zero block sits right before the run loop, so
- * the switch into run mode falls through.  k_RP's step (p ? p - 1 : -2, clamped to
- * [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the carry-in of
- * s_addc from the SCC of p << kR.  The clamp to 128 never binds here: a code
+ * (17) to shift the window, raise k_P by 3, lower k_RP and test the mode
+ * (its lanes decode to 0).  The zero block sits right before the run loop,
+ * so the switch into run mode falls through.  k_RP's step (p ? p - 1 : -2,
+ * clamped to [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the
+ * carry-in of s_addc from the SCC of s_cmp_lg p, 0.  The clamp to 128 never binds here: a code
  * is decoded only when p + 1 + k_R <= 32 (run paths: p + k_R <= 29), and
  * k_RP <= 4 k_R + 3, so the new k_RP is at most 126 - 3p.  Nor does the
  * clamp to 0 on a non-zero k = 0 symbol: p = 0 with u != 0 needs k_R >= 1,
@@ -201,32 +210,33 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
  * of `cur`, a unary prefix of 32+ or a code longer than 32 bits, more than 6
  * one-bits in a run prefix or k > 24 after them, or a zero run reaching the
- * chunk end.  s80-s93 are scratch.  One SGPR written by a scalar
+ * chunk end.  s80-s95 are scratch.  One SGPR written by a scalar
  * instruction and then read as a v_readlane lane select needs 4 wait states
  * (here: three instructions and an s_nop 0). */
 __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint32_t& wpos, uint32_t& kRP,
-                                               uint32_t& kP, uint32_t& neg, uint32_t cur, int32_t& acc) {
+                                               uint32_t& kP, uint32_t& neg, uint32_t cur, uint32_t& aT, uint32_t& aK,
+                                               uint32_t& aF) {
     asm volatile(
+        "s_mov_b64 s[94:95], %[win]\n\t"
         "s_mov_b32 m0, %[neg]\n\t"
         "s_sub_u32 %[kP], %[kP], 4\n\t"
         "s_cbranch_scc0 30f\n\t"
         "10:\n\t"
         /* ---- k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
-        "s_orn2_b64 s[80:81], 1, %[win]\n\t"
+        "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 11f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_addc_u32 s90, %[kRP], s84\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "12:\n\t"
@@ -235,20 +245,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "16:\n\t"
         /* ---- k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
-        "s_orn2_b64 s[80:81], 1, %[win]\n\t"
+        "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 13f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_addc_u32 s90, %[kRP], s84\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "14:\n\t"
@@ -257,20 +266,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "50:\n\t"
         /* ---- k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
-        "s_orn2_b64 s[80:81], 1, %[win]\n\t"
+        "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 51f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_addc_u32 s90, %[kRP], s84\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "52:\n\t"
@@ -279,20 +287,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "54:\n\t"
         /* ---- k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
-        "s_orn2_b64 s[80:81], 1, %[win]\n\t"
+        "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "s_cbranch_scc1 55f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_addc_u32 s90, %[kRP], s84\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "56:\n\t"
@@ -302,15 +309,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "11:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 19f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_addc_u32 s90, %[kRP], s84\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
@@ -320,22 +326,21 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 12b\n\t"
         "13:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 19f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_addc_u32 s90, %[kRP], s84\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
@@ -345,22 +350,21 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 14b\n\t"
         "51:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 19f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_addc_u32 s90, %[kRP], s84\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
@@ -370,22 +374,21 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 52b\n\t"
         "55:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 19f\n\t"
-        "s_lshl_b64 s[80:81], %[win], s84\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "s_lshr_b64 s[82:83], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s87\n\t"
-        "s_lshl_b32 s89, s84, s85\n\t"
-        "s_addc_u32 s90, %[kRP], s84\n\t"
-        "s_or_b32 s89, s89, s82\n\t"
-        "v_writelane_b32 %[acc], s89, m0\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
@@ -395,12 +398,13 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 56b\n\t"
         "18:\n\t"
-        "s_sub_i32 s90, s90, 2\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 88f\n\t"
@@ -409,12 +413,13 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 21f\n\t"
         "17:\n\t"
-        "s_sub_i32 s90, s90, 2\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
         "21:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
@@ -424,15 +429,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 10b\n\t"
         "30:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
-        "s_bitcmp1_b64 %[win], 63\n\t"
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_sub_u32 s88, 62, s85\n\t"
-        "s_lshr_b64 s[90:91], %[win], s88\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 37f\n\t"
         "s_add_u32 s86, s85, 2\n\t"
-        "s_lshl_b64 s[92:93], %[win], s86\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
@@ -440,17 +445,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s86, s86, s89\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 33f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
         "34:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 85f\n\t"
@@ -458,15 +460,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 10b\n\t"
         "40:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
-        "s_bitcmp1_b64 %[win], 63\n\t"
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_sub_u32 s88, 62, s85\n\t"
-        "s_lshr_b64 s[90:91], %[win], s88\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 47f\n\t"
         "s_add_u32 s86, s85, 2\n\t"
-        "s_lshl_b64 s[92:93], %[win], s86\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
@@ -474,17 +476,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s86, s86, s89\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 43f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
         "44:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 85f\n\t"
@@ -492,15 +491,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 10b\n\t"
         "60:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
-        "s_bitcmp1_b64 %[win], 63\n\t"
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_sub_u32 s88, 62, s85\n\t"
-        "s_lshr_b64 s[90:91], %[win], s88\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 67f\n\t"
         "s_add_u32 s86, s85, 2\n\t"
-        "s_lshl_b64 s[92:93], %[win], s86\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
@@ -508,17 +507,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s86, s86, s89\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 63f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
         "64:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 85f\n\t"
@@ -526,15 +522,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 10b\n\t"
         "70:\n\t"
         /* ---- k > 0: run entry, no one-bits ---- */
-        "s_bitcmp1_b64 %[win], 63\n\t"
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
         "s_cbranch_scc1 36f\n\t"
         "s_lshr_b32 s85, %[kP], 2\n\t"
         "s_sub_u32 s88, 62, s85\n\t"
-        "s_lshr_b64 s[90:91], %[win], s88\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 77f\n\t"
         "s_add_u32 s86, s85, 2\n\t"
-        "s_lshl_b64 s[92:93], %[win], s86\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
@@ -542,17 +538,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s86, s86, s89\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 73f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
         "74:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 85f\n\t"
@@ -562,17 +555,14 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "33:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
@@ -580,24 +570,21 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 34b\n\t"
         "43:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
@@ -605,24 +592,21 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 44b\n\t"
         "63:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
@@ -630,24 +614,21 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 64b\n\t"
         "73:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 87f\n\t"
         "s_mov_b32 s91, 0\n\t"
@@ -655,7 +636,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 74b\n\t"
@@ -672,7 +653,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
         "36:\n\t"
-        "s_not_b64 s[80:81], %[win]\n\t"
+        "s_not_b64 s[80:81], s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_mov_b32 s82, 0\n\t"
         "s_add_u32 s87, %[kP], 4\n\t"
@@ -692,7 +673,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 9f\n\t"
         "32:\n\t"
         "s_add_u32 s86, s84, 1\n\t"
-        "s_lshl_b64 s[80:81], %[win], s86\n\t"
+        "s_lshl_b64 s[80:81], s[94:95], s86\n\t"
         "s_sub_u32 s88, 64, s85\n\t"
         "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
         "s_add_u32 s82, s82, s90\n\t"
@@ -709,18 +690,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
         "s_cbranch_scc1 38f\n\t"
         "s_mov_b32 m0, s83\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_u32 %[kP], s87, 5\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "39:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 8f\n\t"
@@ -731,18 +709,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 98f\n\t"
         "s_mov_b32 m0, s83\n\t"
-        "s_lshl_b64 s[80:81], s[92:93], s84\n\t"
-        "s_sub_u32 s88, 63, s89\n\t"
-        "s_lshr_b64 s[90:91], s[80:81], s88\n\t"
-        "s_lshl_b64 %[win], %[win], s86\n\t"
-        "s_lshl_b32 s88, s84, s89\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_or_b32 s88, s88, s90\n\t"
-        "s_add_u32 s88, s88, 1\n\t"
         "s_sub_u32 %[kP], s87, 5\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "v_writelane_b32 %[acc], s88, m0\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
         "s_and_b32 s88, %[wpos], 63\n\t"
         "s_cbranch_scc0 7f\n\t"
         "s_mov_b32 s91, 0\n\t"
@@ -750,7 +725,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_nop 0\n\t"
         "v_readlane_b32 s90, %[cur], s88\n\t"
         "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
-        "s_or_b64 %[win], %[win], s[90:91]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 39b\n\t"
@@ -785,12 +760,26 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_max_i32 %[kP], %[kP], -4\n\t"
         "s_add_u32 %[kP], %[kP], 4\n\t"
         "s_mov_b32 %[neg], m0\n\t"
+        "s_mov_b64 %[win], s[94:95]\n\t"
 
         : [win] "+s"(win), [nbx] "+s"(nbx), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [neg] "+s"(neg),
-          [acc] "+v"(acc)
+          [aT] "+v"(aT), [aK] "+v"(aK), [aF] "+v"(aF)
         : [cur] "v"(cur)
         : "scc", "m0", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92",
-          "s93");
+          "s93", "s94", "s95");
+}
+
+/* The symbol a lane of the scalar loop's accumulators stands for.  K < 32:
+ * T's leading bits are a Golomb-Rice code with parameter K (q one-bits, a
+ * zero, K remainder bits; the scalar loop writes a lane only for codes that
+ * fit in 32 bits), so u = (q << K) | r, plus F (1 for the symbol after a zero
+ * run, stored as u + 1).  K >= 32: T is the value itself.  T = 0 (a lane that
+ * only a zero run passed over) gives 0 for any K < 32. */
+__device__ __forceinline__ uint32_t lane_symbol(uint32_t T, uint32_t K, uint32_t F) {
+    const uint32_t q = static_cast<uint32_t>(__clz(static_cast<int>(~T)));
+    const uint32_t k = K & 31u;
+    const uint32_t r = __builtin_amdgcn_ubfe(T, (31u - q - k) & 31u, k);
+    return (K >= 32u ? T : ((q << k) | r)) + F;
 }
 
 __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src, uint32_t nbytes, uint32_t total,
@@ -800,11 +789,16 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
     br_refill(br);
     br_refill(br);
     const uint32_t lane = lane_id();
-    /* The raw u of symbol acc_base + i sits in lane (i - lim) mod 64, where lim
-     * is the chunk's symbol count (64 for all but a slice's last chunk): the
-     * lane that the scalar loop's v_writelane selects with the low six bits of
-     * its counter nrel - lim.  Full chunks are therefore in lane order. */
-    int32_t acc = 0;
+    /* Symbol acc_base + i sits in lane (i - lim) mod 64, where lim is the
+     * chunk's symbol count (64 for all but a slice's last chunk): the lane that
+     * the scalar loop's v_writelane selects with the low six bits of its
+     * counter nrel - lim.  Full chunks are therefore in lane order.  A lane
+     * holds its symbol in the form lane_symbol() decodes: (aT, aK, aF) =
+     * (the 32 bits that start with the symbol's Golomb-Rice code, k_R, 1 after
+     * a zero run) from the scalar loop, or (raw value, 32, 0) from this
+     * function's own decoding.  Lanes that only a zero run passed over keep
+     * aT = aF = 0, which decodes to 0. */
+    uint32_t aT = 0, aK = 0, aF = 0;
     uint32_t acc_base = 0, nrel = 0;
     uint32_t kP = 0, kRP = 8;
     int st = PVVC_OK;
@@ -813,13 +807,21 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
         const uint32_t r = total - base;
         return r < 64u ? r : 64u;
     };
-    auto store = [&](uint32_t base, int32_t v) {
+    auto store = [&](uint32_t base, uint32_t v) {
         const uint32_t i = base + ((lane + chunk_lim(base)) & 63u);
-        const int32_t s = static_cast<int32_t>(unsign2sign(static_cast<uint32_t>(v)));
+        const int32_t s = static_cast<int32_t>(unsign2sign(v));
         if (i < total) coef[i] = static_cast<float>(static_cast<long long>(s) * static_cast<long long>(qp));
     };
+    auto store_chunk = [&](uint32_t base) {
+        store(base, lane_symbol(aT, aK, aF));
+        aT = 0;
+        aF = 0;
+    };
     auto emit = [&](uint32_t u) {
-        acc = (lane == ((nrel - chunk_lim(acc_base)) & 63u)) ? static_cast<int32_t>(u) : acc;
+        const bool mine = lane == ((nrel - chunk_lim(acc_base)) & 63u);
+        aT = mine ? u : aT;
+        aK = mine ? 32u : aK;
+        aF = mine ? 0u : aF;
         ++nrel;
     };
 
@@ -831,13 +833,12 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
             kP = __builtin_amdgcn_readfirstlane(kP);
             uint32_t neg = __builtin_amdgcn_readfirstlane(nrel - lim);
             int32_t nbx = __builtin_amdgcn_readfirstlane(br.nb - 33);
-            rlgr_fast_run(br.win, nbx, br.wpos, kRP, kP, neg, br.cur, acc);
+            rlgr_fast_run(br.win, nbx, br.wpos, kRP, kP, neg, br.cur, aT, aK, aF);
             nrel = neg + lim;
             br.nb = nbx + 33;
         }
         if (nrel >= lim) { /* chunk of 64 symbols complete */
-            store(acc_base, acc);
-            acc = 0;
+            store_chunk(acc_base);
             acc_base += 64u;
             nrel = 0;
             continue;
@@ -887,16 +888,14 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
         m += br_get(br, k);
         const uint64_t n = static_cast<uint64_t>(acc_base) + nrel;
         if (m >= total - n) {
-            store(acc_base, acc);
-            acc = 0;
+            store_chunk(acc_base);
             acc_base += 64u;
             nrel = 0;
             break;
         }
         const uint32_t n2 = static_cast<uint32_t>(n + m);
         if (n2 - acc_base >= 64u) {
-            store(acc_base, acc);
-            acc = 0;
+            store_chunk(acc_base);
             const uint32_t nb2 = n2 & ~63u;
             for (uint32_t b = acc_base + 64u; b < nb2; b += 64u) store(b, 0);
             acc_base = nb2;
@@ -911,7 +910,7 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
         kRP = adapt_krp(kRP, u >> kR);
         kP = kP ? kP - 1 : 0u;
     }
-    if (acc_base < total) store(acc_base, acc);
+    if (acc_base < total) store_chunk(acc_base);
     for (uint32_t b = acc_base + 64u; b < total; b += 64u) store(b, 0);
     const int64_t consumed = static_cast<int64_t>(br.wpos) * 32 - br.nb;
     if (st == PVVC_OK && consumed > 8ll * nbytes) st = PVVC_ERR_BITSTREAM;

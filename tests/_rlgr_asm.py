@@ -9,9 +9,10 @@ run mode, L = 4, k_P from 0, k_RP from 8), so that a test can check every
 state the loop leaves against the decoder's definition.
 
 Register model: the operands %[win] (64-bit), %[nbx], %[wpos], %[kRP], %[kP],
-%[neg] (32-bit scalars); %[acc] (per-lane vector, read/write), %[cur]
-(per-lane input); s80-s93 and m0 scratch.  Symbol i of a chunk of lim symbols sits
-in accumulator lane (i - lim) mod 64.
+%[neg] (32-bit scalars); %[aT], %[aK], %[aF] (per-lane accumulators,
+read/write), %[cur] (per-lane input); s80-s95 and m0 scratch.  Symbol i of a
+chunk of lim symbols sits in accumulator lane (i - lim) mod 64, in the form
+lane_symbol() decodes.
 """
 from __future__ import annotations
 
@@ -117,7 +118,7 @@ class Loop:
                     nxt = self.target(pc, a[0])
             elif op == "s_nop":
                 pass
-            elif op == "s_mov_b32":
+            elif op in ("s_mov_b32", "s_mov_b64"):
                 wr(a[0], rd(a[1]))
             elif op == "s_not_b64":
                 v = ~rd(a[1]) & M64
@@ -298,13 +299,36 @@ def make_state(bits: list[int], pos: int, kP: int, kRP: int, rng, nb: int, wpos:
                 v = (v << 1) | bits[q + i]
             cur[lane] = v
             q += 32
+    # lanes of the chunk's earlier symbols hold them in the direct form
+    # (value, 32, 0); the others are clear (T = F = 0) with any K
     acc = [int(x) for x in rng.integers(0, 1 << 20, 64)]
+    aT, aF = list(acc), [0] * 64
+    aK = [32] * 64
     for i in range(nrel, 64):
-        acc[(i - lim) & 63] = 0
+        lane = (i - lim) & 63
+        acc[lane] = aT[lane] = 0
+        aK[lane] = (7 * lane + nrel) % 40
     return {
         "win": win, "nbx": (nb - NB_BIAS) & M32, "wpos": wpos & M32, "kRP": kRP, "kP": kP,
-        "neg": (nrel - lim) & M32, "acc": acc, "cur": cur, "lim": lim,
+        "neg": (nrel - lim) & M32, "aT": aT, "aK": aK, "aF": aF, "acc": acc, "cur": cur, "lim": lim,
     }
+
+
+def lane_symbol(T: int, K: int, F: int) -> int:
+    """The symbol one accumulator lane stands for (lane_symbol in
+    pvvc_decode.hip): K < 32: T starts with a Golomb-Rice code of parameter K,
+    u = (q << K) | r, plus F; K >= 32: T itself, plus F."""
+    T &= M32
+    inv = ~T & M32
+    q = 32 - inv.bit_length() if inv else 32
+    k = K & 31
+    r = (T >> ((31 - q - k) & 31)) & ((1 << k) - 1)
+    u = ((q << k) | r) & M32
+    return ((T if (K & M32) >= 32 else u) + F) & M32
+
+
+def lanes(st: dict) -> list[int]:
+    return [lane_symbol(st["aT"][l], st["aK"][l], st["aF"][l]) for l in range(64)]
 
 
 def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> str | None:
@@ -329,8 +353,9 @@ def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> st
     b = Bits(bits, pos)
     kP, kRP, n = st0["kP"], st0["kRP"], nrel0
     acc = list(st0["acc"])
+    got = lanes(st1)
     if consumed == 0:
-        ok = (nrel1 == nrel0 and st1["kP"] == kP and st1["kRP"] == kRP and st1["acc"] == acc)
+        ok = (nrel1 == nrel0 and st1["kP"] == kP and st1["kRP"] == kRP and got == acc)
         return None if ok else "state changed without consuming bits"
     for syms, adv, kP, kRP in events(b, kP, kRP):
         for off, val in syms:
@@ -343,9 +368,9 @@ def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> st
                 return f"symbol index {nrel1} != decoder {n}"
             if st1["kP"] != kP or st1["kRP"] != kRP:
                 return f"kP/kRP {st1['kP']}/{st1['kRP']} != decoder {kP}/{kRP}"
-            if [x & M32 for x in st1["acc"]] != acc:
-                bad = [l for l in range(64) if (st1["acc"][l] & M32) != acc[l]]
-                return f"acc lanes differ: {bad[:8]}"
+            if got != acc:
+                bad = [l for l in range(64) if got[l] != acc[l]]
+                return f"accumulator lanes differ: {bad[:8]}"
             return None
         if used > consumed:
             return f"loop consumed {consumed} bits, not at an event boundary (decoder at {used})"

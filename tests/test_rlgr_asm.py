@@ -77,8 +77,11 @@ def test_scalar_loop_matches_decoder(oracle):
 
 @pytest.mark.parametrize("old,new", [
     ("s_add_u32 %[kP], %[kP], 3", "s_add_u32 %[kP], %[kP], 2"),      # zero symbol: k_P step
-    ("s_sub_u32 s88, 63, s85", "s_sub_u32 s88, 62, s85"),            # remainder width
-    ("s_add_u32 s88, s88, 1", "s_add_u32 s88, s88, 0"),              # run symbol stored as u, not u + 1
+    ("s_sub_u32 s88, 63, s85", "s_sub_u32 s88, 62, s85"),            # zero test over k_R + 1 bits
+    ("v_writelane_b32 %[aF], 1, m0", "v_writelane_b32 %[aF], 0, m0"),  # run symbol stored as u, not u + 1
+    ("v_writelane_b32 %[aK], s85, m0", "v_writelane_b32 %[aK], s84, m0"),  # remainder width k_R
+    ("v_writelane_b32 %[aT], s95, m0", "v_writelane_b32 %[aT], s94, m0"),  # code bits: the window's top
+    ("s_cmp_lg_u32 s84, 0", "s_cmp_eq_u32 s84, 0"),                  # k_RP step: p != 0
     ("s_max_i32 %[kRP], s91, 0", "s_mov_b32 %[kRP], s91"),           # run path: k_RP floor dropped
     ("s_lshr_b32 s85, %[kP], 2", "s_lshr_b32 s85, %[kP], 1"),        # run exponent k = k_P / 4
 ])
@@ -114,6 +117,6 @@ def test_wait_state_check_catches_planted_violations():
     cut = lines[:i] + lines[i + 1:]
     assert any("lane select s88" in b for b in R.hazard_report(cut))
     # an m0 write right before a v_writelane
-    j = lines.index("v_writelane_b32 %[acc], s89, m0")
+    j = lines.index("v_writelane_b32 %[aK], s85, m0")
     moved = lines[:j] + ["s_add_u32 m0, m0, 0"] + lines[j:]
     assert any("lane select m0" in b for b in R.hazard_report(moved))
