@@ -228,9 +228,9 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 11f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
@@ -249,9 +249,9 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 13f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
@@ -270,9 +270,9 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 51f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
@@ -291,9 +291,9 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 55f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 17f\n\t"
@@ -310,7 +310,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 19f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
@@ -334,7 +333,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 19f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
@@ -358,7 +356,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 19f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
@@ -382,7 +379,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 19f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
-        "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
         "s_cbranch_scc0 18f\n\t"
@@ -444,8 +440,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 33f\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 33f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -475,8 +471,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 43f\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 43f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -506,8 +502,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 63f\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 63f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -537,8 +533,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "s_cbranch_scc1 73f\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 73f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -555,7 +551,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "33:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
-        "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -577,7 +572,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "43:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
-        "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -599,7 +593,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "63:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
-        "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -621,7 +614,6 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "73:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
-        "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -730,9 +722,11 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "s_branch 39b\n\t"
         "19:\n\t"
+        "v_writelane_b32 %[aT], 0, m0\n\t"
         "s_add_u32 %[nbx], %[nbx], s87\n\t"
         "s_branch 9f\n\t"
         "97:\n\t"
+        "v_writelane_b32 %[aT], 0, m0\n\t"
         "s_add_u32 %[nbx], %[nbx], s86\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
