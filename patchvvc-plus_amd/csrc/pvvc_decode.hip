@@ -70,15 +70,20 @@ __device__ __forceinline__ void br_init(BitReader& br, const uint8_t* src, uint3
     br.nb = 0;
 }
 
+/* Makes `cur` the 64-dword chunk that holds dword wpos. */
+__device__ __forceinline__ void br_sync(BitReader& br) {
+    if ((br.wpos >> 6) != br.chunk) {
+        br.cur = br.nxt;
+        br.chunk += 1;
+        br.nxt = br_chunk(br, br.chunk + 1);
+    }
+}
+
 /* Appends 32 bits when at most 32 are left, so nb > 32 afterwards. */
 __device__ __forceinline__ void br_refill(BitReader& br) {
     if (br.nb <= 32) {
         const uint32_t l = br.wpos & 63u;
-        if (l == 0 && br.wpos != 0) {
-            br.cur = br.nxt;
-            br.chunk += 1;
-            br.nxt = br_chunk(br, br.chunk + 1);
-        }
+        br_sync(br);
         const uint32_t w = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(br.cur), static_cast<int>(l)));
         br.win |= static_cast<uint64_t>(w) << (32 - br.nb);
         br.nb += 32;
@@ -214,10 +219,12 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * instruction and then read as a v_readlane lane select needs 4 wait states
  * (here: three instructions and an s_nop 0). */
 __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint32_t& wpos, uint32_t& kRP,
-                                               uint32_t& kP, uint32_t& neg, uint32_t cur, uint32_t& aT, uint32_t& aK,
-                                               uint32_t& aF) {
+                                               uint32_t& kP, uint32_t& neg, uint32_t cur, uint32_t pre, uint32_t& aT,
+                                               uint32_t& aK, uint32_t& aF) {
     asm volatile(
         "s_mov_b64 s[94:95], %[win]\n\t"
+        "s_mov_b32 s96, %[pre]\n\t"
+        "s_mov_b32 s97, 0\n\t"
         "s_mov_b32 m0, %[neg]\n\t"
         "s_sub_u32 %[kP], %[kP], 4\n\t"
         "s_cbranch_scc0 30f\n\t"
@@ -318,16 +325,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 7f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 12b\n\t"
         "13:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
@@ -341,16 +347,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 7f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 14b\n\t"
         "51:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
@@ -364,16 +369,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 7f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 52b\n\t"
         "55:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
@@ -387,31 +391,29 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 7f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 56b\n\t"
         "18:\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 88f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 21f\n\t"
         "17:\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
@@ -558,16 +560,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 87f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 34b\n\t"
         "43:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
@@ -579,16 +580,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 87f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 44b\n\t"
         "63:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
@@ -600,16 +600,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 87f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 64b\n\t"
         "73:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
@@ -621,16 +620,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 87f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 74b\n\t"
         "37:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
@@ -710,16 +708,15 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "v_writelane_b32 %[aF], 1, m0\n\t"
-        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
         "s_cbranch_scc0 7f\n\t"
-        "s_mov_b32 s91, 0\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_nop 0\n\t"
-        "v_readlane_b32 s90, %[cur], s88\n\t"
-        "s_lshl_b64 s[90:91], s[90:91], s89\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 39b\n\t"
         "19:\n\t"
         "v_writelane_b32 %[aT], 0, m0\n\t"
@@ -758,9 +755,9 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
 
         : [win] "+s"(win), [nbx] "+s"(nbx), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [neg] "+s"(neg),
           [aT] "+v"(aT), [aK] "+v"(aK), [aF] "+v"(aF)
-        : [cur] "v"(cur)
+        : [cur] "v"(cur), [pre] "s"(pre)
         : "scc", "m0", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92",
-          "s93", "s94", "s95");
+          "s93", "s94", "s95", "s96", "s97");
 }
 
 /* The symbol a lane of the scalar loop's accumulators stands for.  K < 32:
@@ -804,7 +801,10 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
     auto store = [&](uint32_t base, uint32_t v) {
         const uint32_t i = base + ((lane + chunk_lim(base)) & 63u);
         const int32_t s = static_cast<int32_t>(unsign2sign(v));
-        if (i < total) coef[i] = static_cast<float>(static_cast<long long>(s) * static_cast<long long>(qp));
+        /* (float)(int64 s * qp) with one rounding: the FP64 product is exact
+         * (|s * qp| < 2^39), the FP64 -> FP32 conversion rounds to nearest even
+         * like the integer one */
+        if (i < total) coef[i] = static_cast<float>(static_cast<double>(s) * static_cast<double>(qp));
     };
     auto store_chunk = [&](uint32_t base) {
         store(base, lane_symbol(aT, aK, aF));
@@ -827,7 +827,12 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
             kP = __builtin_amdgcn_readfirstlane(kP);
             uint32_t neg = __builtin_amdgcn_readfirstlane(nrel - lim);
             int32_t nbx = __builtin_amdgcn_readfirstlane(br.nb - 33);
-            rlgr_fast_run(br.win, nbx, br.wpos, kRP, kP, neg, br.cur, aT, aK, aF);
+            /* the loop takes the next dword (the window's next refill) in
+             * an SGPR and reads the one after it during each refill */
+            br_sync(br);
+            const uint32_t pre =
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(br.cur), static_cast<int>(br.wpos & 63u)));
+            rlgr_fast_run(br.win, nbx, br.wpos, kRP, kP, neg, br.cur, pre, aT, aK, aF);
             nrel = neg + lim;
             br.nb = nbx + 33;
         }

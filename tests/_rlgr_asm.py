@@ -10,7 +10,8 @@ state the loop leaves against the decoder's definition.
 
 Register model: the operands %[win] (64-bit), %[nbx], %[wpos], %[kRP], %[kP],
 %[neg] (32-bit scalars); %[aT], %[aK], %[aF] (per-lane accumulators,
-read/write), %[cur] (per-lane input); s80-s95 and m0 scratch.  Symbol i of a
+read/write), %[cur] (per-lane input), %[pre] (its dword wpos); s80-s97 and
+m0 scratch.  Symbol i of a
 chunk of lim symbols sits in accumulator lane (i - lim) mod 64, in the form
 lane_symbol() decodes.
 """
@@ -284,7 +285,8 @@ def decode_boundaries(bits: list[int], nsym: int):
 
 def make_state(bits: list[int], pos: int, kP: int, kRP: int, rng, nb: int, wpos: int, lim: int, nrel: int):
     """Loop entry state at bit `pos`: nb window bits, then the stream's dwords
-    in lanes wpos & 63 .. 63 of `cur` (lane 0 would be a chunk rotation)."""
+    in lanes wpos & 63 .. 63 of `cur` (the chunk that holds dword wpos), the
+    first of them also in `pre`."""
     win = 0
     for i in range(nb):
         win = (win << 1) | bits[pos + i]
@@ -292,13 +294,12 @@ def make_state(bits: list[int], pos: int, kP: int, kRP: int, rng, nb: int, wpos:
     l0 = wpos & 63
     cur = [int(x) for x in rng.integers(0, 1 << 32, 64, dtype="uint64")]
     q = pos + nb
-    if l0:
-        for lane in range(l0, 64):
-            v = 0
-            for i in range(32):
-                v = (v << 1) | bits[q + i]
-            cur[lane] = v
-            q += 32
+    for lane in range(l0, 64):
+        v = 0
+        for i in range(32):
+            v = (v << 1) | bits[q + i]
+        cur[lane] = v
+        q += 32
     # lanes of the chunk's earlier symbols hold them in the direct form
     # (value, 32, 0); the others are clear (T = F = 0) with any K
     acc = [int(x) for x in rng.integers(0, 1 << 20, 64)]
@@ -310,7 +311,8 @@ def make_state(bits: list[int], pos: int, kP: int, kRP: int, rng, nb: int, wpos:
         aK[lane] = (7 * lane + nrel) % 40
     return {
         "win": win, "nbx": (nb - NB_BIAS) & M32, "wpos": wpos & M32, "kRP": kRP, "kP": kP,
-        "neg": (nrel - lim) & M32, "aT": aT, "aK": aK, "aF": aF, "acc": acc, "cur": cur, "lim": lim,
+        "neg": (nrel - lim) & M32, "aT": aT, "aK": aK, "aF": aF, "acc": acc, "cur": cur, "pre": cur[l0],
+        "lim": lim,
     }
 
 

@@ -107,15 +107,17 @@ def test_loop_wait_states():
 
 def test_wait_state_check_catches_planted_violations():
     """The checker fails loops whose lane select is written too close to its
-    use: s88 recomputed two instructions before a refill's v_readlane, a
-    refill without its s_nop, or an m0 update right before a v_writelane."""
+    use: s88 recomputed two instructions before a refill's v_readlane, the
+    v_readlane moved up to right after the s_and that computes its lane, or an
+    m0 update right before a v_writelane."""
     lines = R.extract_loop()
-    i = lines.index("s_nop 0")
-    close = lines[:i - 1] + ["s_and_b32 s88, %[wpos], 63"] + lines[i - 1:]
+    i = lines.index("v_readlane_b32 s96, %[cur], s88")
+    close = lines[:i - 1] + ["s_and_b32 s88, s88, 63"] + lines[i - 1:]
     assert any("lane select s88" in b for b in R.hazard_report(close))
-    # the refill's s_nop is the fourth wait state: without it the check fails
-    cut = lines[:i] + lines[i + 1:]
-    assert any("lane select s88" in b for b in R.hazard_report(cut))
+    # the refill computes its lane (s_and) six instructions before the read
+    j = max(k for k in range(i) if lines[k] == "s_and_b32 s88, s88, 63")
+    early = lines[:j + 1] + [lines[i]] + lines[j + 1:i] + lines[i + 1:]
+    assert any("lane select s88" in b for b in R.hazard_report(early))
     # an m0 write right before a v_writelane
     j = lines.index("v_writelane_b32 %[aK], s85, m0")
     moved = lines[:j] + ["s_add_u32 m0, m0, 0"] + lines[j:]
