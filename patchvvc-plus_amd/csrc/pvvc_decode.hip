@@ -2436,17 +2436,24 @@ int plan_batch(const pvvc_frame* const* frames, int n_frames, const int* slot_ba
             }
             pmax[sl] = std::max(pmax[sl], p.items[i].n);
         }
-        /* key: larger first, intra first on ties, then item order */
-        std::vector<uint64_t> keys(hi - lo);
+        /* key: larger first; on ties a patch's slices together (they read
+         * the same intra geometry), its intra slice first, then item order */
+        struct Key {
+            uint64_t hi;  /* size (descending) | patch */
+            uint32_t lo;  /* predictive flag | item */
+            bool operator<(const Key& o) const { return hi != o.hi ? hi < o.hi : lo < o.lo; }
+        };
+        std::vector<Key> keys(hi - lo);
         for (uint32_t i = lo; i < hi; ++i) {
             const ItemDev& A = p.items[i];
             const bool pa = A.type & PVVC_TYPE_PREDICT;
             const uint32_t ka = pa ? A.n : pmax[A.patch];
-            keys[i - lo] = (static_cast<uint64_t>(0xffffffffu - ka) << 32) | (static_cast<uint64_t>(pa) << 31) | i;
+            keys[i - lo] = Key{(static_cast<uint64_t>(0xffffffffu - ka) << 32) | static_cast<uint32_t>(A.patch),
+                               (static_cast<uint32_t>(pa) << 31) | i};
         }
         std::sort(keys.begin(), keys.end());
         std::vector<uint32_t> ids(hi - lo);
-        for (uint32_t j = 0; j < hi - lo; ++j) ids[j] = static_cast<uint32_t>(keys[j] & 0x7fffffffu);
+        for (uint32_t j = 0; j < hi - lo; ++j) ids[j] = keys[j].lo & 0x7fffffffu;
         uint64_t need = 0;
         for (uint32_t i : ids) {
             const ItemDev& x = p.items[i];
