@@ -1097,9 +1097,10 @@ __device__ __forceinline__ void inv_haar(uint32_t w0, uint32_t w1, float g, floa
 /* ======================================================================== */
 
 /* Optional per-phase time stamps (built with -DPVVC_PHASE_PROFILE): for
- * item i, stamps [8i, 8i+8) = start, after tree, after RLGR, after the
- * intra wait, after the top-down pass, end, and the XCC id; device-wide
- * 100 MHz clock (s_memrealtime). */
+ * item i, stamps [8i, 8i+8) = start, after RLGR (one wavefront per slice;
+ * unset otherwise), after the tree (likewise), after RLGR and tree, after
+ * the top-down pass, end, and the hardware id; device-wide 100 MHz clock
+ * (s_memrealtime). */
 #ifdef PVVC_PHASE_PROFILE
 __device__ uint64_t* g_prof = nullptr;
 #define PVVC_STAMP(item, k)                                              \
@@ -1279,12 +1280,14 @@ __device__ __forceinline__ int decode_slice(SliceArgs a, SlotDev* slot, Arena ar
     /* ---- R (RLGR) and T (tree) ---- */
     if (st == PVVC_OK) {
         if (WAVES == 1) {
-            int ts = tree_structure(a.geo, a.geo_size, a.n, fc, meta);
+            /* RLGR first: a launch's first slices then all start on the
+             * scalar unit, instead of all building their trees (vector and
+             * memory work) while it idles; a tree that fails wins the status */
+            const int rs = pre_coef ? pre_status[item] : rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
+            PVVC_STAMP(item, 1);
+            const int ts = tree_structure(a.geo, a.geo_size, a.n, fc, meta);
             __syncthreads();
             if (ts == PVVC_OK) tree_weights(a.geo, fc, wt, meta);
-            PVVC_STAMP(item, 1);
-            int rs = PVVC_OK;
-            if (ts == PVVC_OK) rs = pre_coef ? pre_status[item] : rlgr_decode_wave(a.col, a.col_size, 3u * a.n, a.qp, coef);
             PVVC_STAMP(item, 2);
             st = ts != PVVC_OK ? ts : rs;
         } else {
