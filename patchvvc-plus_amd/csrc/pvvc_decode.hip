@@ -1550,10 +1550,20 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
     __shared__ LevelMeta meta;
     __shared__ uint32_t next;
     uint8_t* ws = slab + static_cast<uint64_t>(blockIdx.x) * slab_bytes;
+    /* a workgroup's first entries are its own (blockIdx.x * chunk), so every
+     * workgroup starts decoding without an atomic round trip; the shared
+     * counter hands out the entries after the first grid * chunk, one atomic
+     * per `chunk` consecutive entries */
+    const uint32_t first_end = gridDim.x * chunk;
+    bool first = true;
     for (;;) {
-        /* one atomic per `chunk` consecutive queue entries */
-        if (threadIdx.x == 0) next = atomicAdd(queue, chunk);
-        __syncthreads();
+        if (first) {
+            next = blockIdx.x * chunk;
+            first = false;
+        } else {
+            if (threadIdx.x == 0) next = first_end + atomicAdd(queue, chunk);
+            __syncthreads();
+        }
         const uint32_t q0 = next;
         __syncthreads();
         if (q0 >= count) break;
