@@ -172,16 +172,24 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  *    refill follows, so that one test stands for both; a borrow leaves the
  *    line for a block that tests for a code over 32 bits (an exit, nbx
  *    restored), then finishes the symbol and refills.
- * The k = 0 and run cases are two loops, each unrolled four times (k = 0:
- * 10, 16, 50, 54; run: 30, 40, 60, 70) and handing over to the other when
- * k_P crosses 3, so that the common path falls through: a taken scalar
+ * k_P is not held in a register: it is the position in the code.  One
+ * block per k_P value decodes one symbol with k = k_P / 4 and the shift
+ * counts of its run entry as constants: G (k_P >= 12, the only one that
+ * keeps k_P in its register), R11..R8 (k = 2), R7..R4 (k = 1), S3..S1 and
+ * four copies of S0 (k = 0), laid out in that order.  A symbol that lowers
+ * k_P by one (a run symbol; a non-zero k = 0 symbol, floor 0) falls through
+ * into the next block, so the common path takes no branch: a taken scalar
  * branch costs a wavefront ~30 cycles against ~4 for an instruction, and a
- * single slice's decode (the drop-in launch) is bound by that latency.  A
- * non-zero k = 0 symbol lowers k_P, so it stays in the k = 0 loop without a
- * mode test; a zero symbol (u = 0) leaves the line for the one zero block
- * (17) to shift the window, raise k_P by 3, lower k_RP and test the mode
- * (its lanes decode to 0).  The zero block sits right before the run loop,
- * so the switch into run mode falls through.  k_RP's step (p ? p - 1 : -2,
+ * single slice's decode (the drop-in launch) is bound by that latency; S0d
+ * jumps back to S0a.  A zero k = 0 symbol (u = 0) leaves the line for its
+ * state's zero block (560-563), which shifts the window, lowers k_RP and
+ * jumps to the block of k_P + 3 (S3, or R4..R6: run mode); its lanes decode
+ * to 0.  A run prefix with one-bits (36) ends with k_P in its register and
+ * enters its block through the dispatch (91) that also starts the loop.
+ * Every exit first writes k_P (the stubs 600-723).  Per symbol this saves
+ * the k_P step and its mode test, and on the run path the exponent and two
+ * shift counts: 14.2 -> 12.7 scalar instructions per symbol on c1 colour
+ * streams.  k_RP's step (p ? p - 1 : -2,
  * clamped to [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the
  * carry-in of s_addc from the SCC of s_cmp_lg p, 0.  The clamp to 128 never binds here: a code
  * is decoded only when p + 1 + k_R <= 32 (run paths: p + k_R <= 29), and
@@ -189,35 +197,28 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * clamp to 0 on a non-zero k = 0 symbol: p = 0 with u != 0 needs k_R >= 1,
  * i.e. k_RP >= 4.  Only the zero symbols and the run paths (where u = 0 is a
  * valid symbol, 1) keep it.  A run entry whose prefix has no one-bits
- * (the common case) has its own path in each run copy: its zero count is
- * the k-bit suffix alone and `neg` advances in place (undone on the
- * out-of-line exits 37/47/67/77); prefixes with one-bits take the one general
- * path, laid out as entry (36), one-bits loop (31/35) and symbol (32) in a
- * line; a one-bit test of the window's top bit chooses between them.  The
+ * (the common case) is the run block itself: its zero count is the k-bit
+ * suffix alone and `neg` advances in place (undone on the out-of-line exits
+ * 684-691, 722); prefixes with one-bits take the one general path, laid out
+ * as entry (36), one-bits loop (31/35) and symbol (32) in a line; a one-bit
+ * test of the window's top bit chooses between them.  The
  * leading-ones count of a Golomb code is s_flbit of s_orn2(1, window): that
  * value is never 0, so the count stays below 64 and s_orn2 sets SCC, which
  * s_flbit leaves for the s_addc that adds the code's terminating zero.  A
  * run's zero count is read straight from the window (its top bit is the
- * prefix's terminating zero).  Inside the loop k_P is held as k_P - 4 (set
- * on entry, restored at the exit 9), so that the update itself is the mode
- * test: a zero symbol's +3 carries exactly when k_P reaches 4 (run mode),
- * and a run symbol's -1 borrows exactly when it falls to 3 (k = 0 mode);
- * those updates follow the chunk test, so their chunk-end and rotation exits
- * take them on the way out (85-88).  A non-zero k = 0 symbol's -1 is not
- * clamped: k_P - 4 may fall below -4 in the k = 0 loop (by at most one chunk
- * of symbols) and is clamped where it is read, in the zero block and at the
- * exits.  Symbols that end in a refill finish out of line (11/13/51/55, 18,
- * 33/43/63/73, 38).
+ * prefix's terminating zero).  Symbols that end in a refill finish out of
+ * line, one block per state copy (block + 400, 580-583, 38).
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window; every exit leaves the state of the last whole symbol (the
- * early nbx and m0 updates are undone on the way out: 19, 97, 98 and
- * 37/47/67/77).  The loop returns
+ * early nbx and m0 updates are undone on the way out: 19, 97, 98, 684-691
+ * and 722).  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
  * of `cur`, a unary prefix of 32+ or a code longer than 32 bits, more than 6
  * one-bits in a run prefix or k > 24 after them, or a zero run reaching the
- * chunk end.  s80-s95 are scratch.  One SGPR written by a scalar
- * instruction and then read as a v_readlane lane select needs 4 wait states
- * (here: three instructions and an s_nop 0). */
+ * chunk end.  s80-s97 are scratch.  An SGPR written by a scalar
+ * instruction and then read as a v_readlane / v_writelane lane select gets
+ * 4 wait states (here at least six instructions on every path;
+ * tests/test_rlgr_asm.py checks the distances on the loop text). */
 __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint32_t& wpos, uint32_t& kRP,
                                                uint32_t& kP, uint32_t& neg, uint32_t cur, uint32_t pre, uint32_t& aT,
                                                uint32_t& aK, uint32_t& aF) {
@@ -226,224 +227,655 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_mov_b32 s96, %[pre]\n\t"
         "s_mov_b32 s97, 0\n\t"
         "s_mov_b32 m0, %[neg]\n\t"
-        "s_sub_u32 %[kP], %[kP], 4\n\t"
-        "s_cbranch_scc0 30f\n\t"
-        "10:\n\t"
-        /* ---- k = 0 symbol ---- */
+        "s_cmp_lt_u32 %[kP], 4\n\t"
+        "s_cbranch_scc1 93f\n\t"
+        "91:\n\t"
+        "s_cmp_lt_u32 %[kP], 8\n\t"
+        "s_cbranch_scc0 92f\n\t"
+        "s_cmp_eq_u32 %[kP], 5\n\t"
+        "s_cbranch_scc1 105f\n\t"
+        "s_cmp_eq_u32 %[kP], 6\n\t"
+        "s_cbranch_scc1 106f\n\t"
+        "s_cmp_eq_u32 %[kP], 7\n\t"
+        "s_cbranch_scc1 107f\n\t"
+        "s_branch 104f\n\t"
+        "92:\n\t"
+        "s_cmp_lt_u32 %[kP], 12\n\t"
+        "s_cbranch_scc0 100f\n\t"
+        "s_cmp_eq_u32 %[kP], 8\n\t"
+        "s_cbranch_scc1 108f\n\t"
+        "s_cmp_eq_u32 %[kP], 9\n\t"
+        "s_cbranch_scc1 109f\n\t"
+        "s_cmp_eq_u32 %[kP], 10\n\t"
+        "s_cbranch_scc1 110f\n\t"
+        "s_branch 111f\n\t"
+        "93:\n\t"
+        "s_cmp_eq_u32 %[kP], 0\n\t"
+        "s_cbranch_scc1 120f\n\t"
+        "s_cmp_eq_u32 %[kP], 1\n\t"
+        "s_cbranch_scc1 121f\n\t"
+        "s_cmp_eq_u32 %[kP], 2\n\t"
+        "s_cbranch_scc1 122f\n\t"
+        "s_branch 123f\n\t"
+        "100:\n\t"
+        /* ---- G: k >= 3, k_P in its register ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 723f\n\t"
+        "s_lshr_b32 s85, %[kP], 2\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 722f\n\t"
+        "s_add_u32 s86, s85, 1\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s86, s84\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 500f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "300:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 720f\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_cmp_lt_u32 %[kP], 12\n\t"
+        "s_cbranch_scc0 100b\n\t"
+        "111:\n\t"
+        /* ---- R11: k_P = 11, run entry without one-bits ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 711f\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 691f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 3\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 511f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "311:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 610f\n\t"
+        "110:\n\t"
+        /* ---- R10: k_P = 10, run entry without one-bits ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 710f\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 690f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 3\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 510f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "310:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 609f\n\t"
+        "109:\n\t"
+        /* ---- R9: k_P = 9, run entry without one-bits ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 709f\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 689f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 3\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 509f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "309:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 608f\n\t"
+        "108:\n\t"
+        /* ---- R8: k_P = 8, run entry without one-bits ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 708f\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 688f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 3\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 508f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "308:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 607f\n\t"
+        "107:\n\t"
+        /* ---- R7: k_P = 7, run entry without one-bits ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 707f\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 687f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 2\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 2\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 507f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "307:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 606f\n\t"
+        "106:\n\t"
+        /* ---- R6: k_P = 6, run entry without one-bits ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 706f\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 686f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 2\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 2\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 506f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "306:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 605f\n\t"
+        "105:\n\t"
+        /* ---- R5: k_P = 5, run entry without one-bits ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 705f\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 685f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 2\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 2\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 505f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "305:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 604f\n\t"
+        "104:\n\t"
+        /* ---- R4: k_P = 4, run entry without one-bits ---- */
+        "s_bitcmp1_b64 s[94:95], 63\n\t"
+        "s_cbranch_scc1 704f\n\t"
+        "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 684f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 2\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 2\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 504f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "304:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 603f\n\t"
+        "123:\n\t"
+        /* ---- S3: k_P = 3, k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
-        "s_cbranch_scc1 11f\n\t"
+        "s_cbranch_scc1 523f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
-        "s_cbranch_scc0 17f\n\t"
+        "s_cbranch_scc0 563f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "12:\n\t"
+        "323:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "16:\n\t"
-        /* ---- k = 0 symbol ---- */
+        "s_cbranch_scc1 602f\n\t"
+        "122:\n\t"
+        /* ---- S2: k_P = 2, k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
-        "s_cbranch_scc1 13f\n\t"
+        "s_cbranch_scc1 522f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
-        "s_cbranch_scc0 17f\n\t"
+        "s_cbranch_scc0 562f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "14:\n\t"
+        "322:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "50:\n\t"
-        /* ---- k = 0 symbol ---- */
+        "s_cbranch_scc1 601f\n\t"
+        "121:\n\t"
+        /* ---- S1: k_P = 1, k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
-        "s_cbranch_scc1 51f\n\t"
+        "s_cbranch_scc1 521f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
-        "s_cbranch_scc0 17f\n\t"
+        "s_cbranch_scc0 561f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "52:\n\t"
+        "321:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "54:\n\t"
-        /* ---- k = 0 symbol ---- */
+        "s_cbranch_scc1 600f\n\t"
+        "120:\n\t"
+        /* ---- S0: k_P = 0, k = 0 symbol ---- */
         "s_lshr_b32 s85, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
         "s_sub_u32 %[nbx], %[nbx], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
-        "s_cbranch_scc1 55f\n\t"
+        "s_cbranch_scc1 520f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
-        "s_cbranch_scc0 17f\n\t"
+        "s_cbranch_scc0 560f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "56:\n\t"
+        "320:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_branch 10b\n\t"
-        "11:\n\t"
-        "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 19f\n\t"
+        "s_cbranch_scc1 600f\n\t"
+        "130:\n\t"
+        /* ---- S0: k_P = 0, k = 0 symbol ---- */
+        "s_lshr_b32 s85, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s87, s84, s85\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_cbranch_scc1 530f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
-        "s_cbranch_scc0 18f\n\t"
+        "s_cbranch_scc0 560f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "330:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 600f\n\t"
+        "140:\n\t"
+        /* ---- S0: k_P = 0, k = 0 symbol ---- */
+        "s_lshr_b32 s85, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s87, s84, s85\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_cbranch_scc1 540f\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
+        "s_cbranch_scc0 560f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "340:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 600f\n\t"
+        "150:\n\t"
+        /* ---- S0: k_P = 0, k = 0 symbol ---- */
+        "s_lshr_b32 s85, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s87, s84, s85\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "v_writelane_b32 %[aT], s95, m0\n\t"
+        "s_cbranch_scc1 550f\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
+        "s_cbranch_scc0 560f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "350:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 600f\n\t"
+        "s_branch 120b\n\t"
+        "560:\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_sub_i32 s90, %[kRP], 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "570:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 603f\n\t"
+        "s_branch 123b\n\t"
+        "561:\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_sub_i32 s90, %[kRP], 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "571:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 604f\n\t"
+        "s_branch 104b\n\t"
+        "562:\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_sub_i32 s90, %[kRP], 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "572:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 605f\n\t"
+        "s_branch 105b\n\t"
+        "563:\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_sub_i32 s90, %[kRP], 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "573:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 606f\n\t"
+        "s_branch 106b\n\t"
+        "523:\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 643f\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
+        "s_cbranch_scc0 583f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
         "s_add_u32 s88, %[wpos], 1\n\t"
         "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 7f\n\t"
+        "s_cbranch_scc0 622f\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 12b\n\t"
-        "13:\n\t"
+        "s_branch 323b\n\t"
+        "522:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 19f\n\t"
+        "s_cbranch_scc1 642f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
-        "s_cbranch_scc0 18f\n\t"
+        "s_cbranch_scc0 582f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_add_u32 s88, %[wpos], 1\n\t"
         "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 7f\n\t"
+        "s_cbranch_scc0 621f\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 14b\n\t"
-        "51:\n\t"
+        "s_branch 322b\n\t"
+        "521:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 19f\n\t"
+        "s_cbranch_scc1 641f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
-        "s_cbranch_scc0 18f\n\t"
+        "s_cbranch_scc0 581f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_add_u32 s88, %[wpos], 1\n\t"
         "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 7f\n\t"
+        "s_cbranch_scc0 620f\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 52b\n\t"
-        "55:\n\t"
+        "s_branch 321b\n\t"
+        "520:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
-        "s_cbranch_scc1 19f\n\t"
+        "s_cbranch_scc1 640f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
         "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
         "v_writelane_b32 %[aK], s85, m0\n\t"
-        "s_cbranch_scc0 18f\n\t"
+        "s_cbranch_scc0 580f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
         "s_add_u32 s88, %[wpos], 1\n\t"
         "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 7f\n\t"
+        "s_cbranch_scc0 620f\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 56b\n\t"
-        "18:\n\t"
+        "s_branch 320b\n\t"
+        "530:\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 640f\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
+        "s_cbranch_scc0 580f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 620f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 330b\n\t"
+        "540:\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 640f\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
+        "s_cbranch_scc0 580f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 620f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 340b\n\t"
+        "550:\n\t"
+        "s_cmp_gt_u32 s87, 32\n\t"
+        "s_cbranch_scc1 640f\n\t"
+        "s_sub_u32 s88, 63, s85\n\t"
+        "s_lshr_b64 s[82:83], s[94:95], s88\n\t"
+        "v_writelane_b32 %[aK], s85, m0\n\t"
+        "s_cbranch_scc0 580f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "s_addc_u32 s90, %[kRP], s84\n\t"
+        "s_sub_i32 %[kRP], s90, 2\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 620f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 350b\n\t"
+        "580:\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
         "s_add_u32 s88, %[wpos], 1\n\t"
         "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 88f\n\t"
+        "s_cbranch_scc0 623f\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 21f\n\t"
-        "17:\n\t"
+        "s_branch 570b\n\t"
+        "581:\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
-        "21:\n\t"
-        "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 86f\n\t"
-        "s_max_i32 %[kP], %[kP], -4\n\t"
-        "s_add_u32 %[kP], %[kP], 3\n\t"
-        "s_cbranch_scc0 10b\n\t"
-        "30:\n\t"
-        /* ---- k > 0: run entry, no one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 36f\n\t"
-        "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_sub_u32 s88, 62, s85\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
-        "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 37f\n\t"
-        "s_add_u32 s86, s85, 2\n\t"
-        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
-        "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_addc_u32 s86, s86, s84\n\t"
-        "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "v_writelane_b32 %[aT], s93, m0\n\t"
-        "s_cbranch_scc1 33f\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 624f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 571b\n\t"
+        "582:\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_sub_i32 s90, %[kRP], 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 625f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 572b\n\t"
+        "583:\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
+        "s_sub_i32 s90, %[kRP], 2\n\t"
+        "s_max_i32 %[kRP], s90, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 626f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 573b\n\t"
+        "511:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 671f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -451,30 +883,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "34:\n\t"
-        "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 85f\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_cbranch_scc1 10b\n\t"
-        "40:\n\t"
-        /* ---- k > 0: run entry, no one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 36f\n\t"
-        "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_sub_u32 s88, 62, s85\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
-        "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 47f\n\t"
-        "s_add_u32 s86, s85, 2\n\t"
-        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
-        "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_addc_u32 s86, s86, s84\n\t"
-        "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "v_writelane_b32 %[aT], s93, m0\n\t"
-        "s_cbranch_scc1 43f\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 630f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 311b\n\t"
+        "510:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 670f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -482,30 +903,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "44:\n\t"
-        "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 85f\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_cbranch_scc1 10b\n\t"
-        "60:\n\t"
-        /* ---- k > 0: run entry, no one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 36f\n\t"
-        "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_sub_u32 s88, 62, s85\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
-        "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 67f\n\t"
-        "s_add_u32 s86, s85, 2\n\t"
-        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
-        "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_addc_u32 s86, s86, s84\n\t"
-        "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "v_writelane_b32 %[aT], s93, m0\n\t"
-        "s_cbranch_scc1 63f\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 629f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 310b\n\t"
+        "509:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 669f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -513,30 +923,19 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "64:\n\t"
-        "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 85f\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_cbranch_scc1 10b\n\t"
-        "70:\n\t"
-        /* ---- k > 0: run entry, no one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 36f\n\t"
-        "s_lshr_b32 s85, %[kP], 2\n\t"
-        "s_sub_u32 s88, 62, s85\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], s88\n\t"
-        "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 77f\n\t"
-        "s_add_u32 s86, s85, 2\n\t"
-        "s_lshl_b64 s[92:93], s[94:95], s86\n\t"
-        "s_lshr_b32 s89, %[kRP], 2\n\t"
-        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
-        "s_flbit_i32_b64 s84, s[80:81]\n\t"
-        "s_addc_u32 s86, s86, s84\n\t"
-        "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
-        "v_writelane_b32 %[aT], s93, m0\n\t"
-        "s_cbranch_scc1 73f\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 628f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 309b\n\t"
+        "508:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 668f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
@@ -544,13 +943,97 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "74:\n\t"
-        "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 85f\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_cbranch_scc1 10b\n\t"
-        "s_branch 30b\n\t"
-        "33:\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 627f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 308b\n\t"
+        "507:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 667f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 626f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 307b\n\t"
+        "506:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 666f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 625f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 306b\n\t"
+        "505:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 665f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 624f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 305b\n\t"
+        "504:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 664f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 623f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 304b\n\t"
+        "500:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -562,91 +1045,196 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_max_i32 %[kRP], s91, 0\n\t"
         "s_add_u32 s88, %[wpos], 1\n\t"
         "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 87f\n\t"
+        "s_cbranch_scc0 721f\n\t"
         "s_sub_u32 s89, -1, %[nbx]\n\t"
         "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
         "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
         "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 34b\n\t"
-        "43:\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 97f\n\t"
-        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
-        "v_writelane_b32 %[aK], s89, m0\n\t"
-        "s_cmp_lg_u32 s84, 0\n\t"
-        "v_writelane_b32 %[aF], 1, m0\n\t"
-        "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_sub_i32 s91, s91, 2\n\t"
-        "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 87f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
-        "s_add_u32 %[wpos], %[wpos], 1\n\t"
-        "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 44b\n\t"
-        "63:\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 97f\n\t"
-        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
-        "v_writelane_b32 %[aK], s89, m0\n\t"
-        "s_cmp_lg_u32 s84, 0\n\t"
-        "v_writelane_b32 %[aF], 1, m0\n\t"
-        "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_sub_i32 s91, s91, 2\n\t"
-        "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 87f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
-        "s_add_u32 %[wpos], %[wpos], 1\n\t"
-        "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 64b\n\t"
-        "73:\n\t"
-        "s_cmp_gt_u32 s86, 32\n\t"
-        "s_cbranch_scc1 97f\n\t"
-        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
-        "v_writelane_b32 %[aK], s89, m0\n\t"
-        "s_cmp_lg_u32 s84, 0\n\t"
-        "v_writelane_b32 %[aF], 1, m0\n\t"
-        "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_sub_i32 s91, s91, 2\n\t"
-        "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 87f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
-        "s_add_u32 %[wpos], %[wpos], 1\n\t"
-        "v_readlane_b32 s96, %[cur], s88\n\t"
-        "s_branch 74b\n\t"
-        "37:\n\t"
+        "s_branch 300b\n\t"
+        "600:\n\t"
+        "s_mov_b32 %[kP], 0\n\t"
+        "s_branch 9f\n\t"
+        "601:\n\t"
+        "s_mov_b32 %[kP], 1\n\t"
+        "s_branch 9f\n\t"
+        "602:\n\t"
+        "s_mov_b32 %[kP], 2\n\t"
+        "s_branch 9f\n\t"
+        "603:\n\t"
+        "s_mov_b32 %[kP], 3\n\t"
+        "s_branch 9f\n\t"
+        "604:\n\t"
+        "s_mov_b32 %[kP], 4\n\t"
+        "s_branch 9f\n\t"
+        "605:\n\t"
+        "s_mov_b32 %[kP], 5\n\t"
+        "s_branch 9f\n\t"
+        "606:\n\t"
+        "s_mov_b32 %[kP], 6\n\t"
+        "s_branch 9f\n\t"
+        "607:\n\t"
+        "s_mov_b32 %[kP], 7\n\t"
+        "s_branch 9f\n\t"
+        "608:\n\t"
+        "s_mov_b32 %[kP], 8\n\t"
+        "s_branch 9f\n\t"
+        "609:\n\t"
+        "s_mov_b32 %[kP], 9\n\t"
+        "s_branch 9f\n\t"
+        "610:\n\t"
+        "s_mov_b32 %[kP], 10\n\t"
+        "s_branch 9f\n\t"
+        "620:\n\t"
+        "s_mov_b32 %[kP], 0\n\t"
+        "s_branch 7f\n\t"
+        "621:\n\t"
+        "s_mov_b32 %[kP], 1\n\t"
+        "s_branch 7f\n\t"
+        "622:\n\t"
+        "s_mov_b32 %[kP], 2\n\t"
+        "s_branch 7f\n\t"
+        "623:\n\t"
+        "s_mov_b32 %[kP], 3\n\t"
+        "s_branch 7f\n\t"
+        "624:\n\t"
+        "s_mov_b32 %[kP], 4\n\t"
+        "s_branch 7f\n\t"
+        "625:\n\t"
+        "s_mov_b32 %[kP], 5\n\t"
+        "s_branch 7f\n\t"
+        "626:\n\t"
+        "s_mov_b32 %[kP], 6\n\t"
+        "s_branch 7f\n\t"
+        "627:\n\t"
+        "s_mov_b32 %[kP], 7\n\t"
+        "s_branch 7f\n\t"
+        "628:\n\t"
+        "s_mov_b32 %[kP], 8\n\t"
+        "s_branch 7f\n\t"
+        "629:\n\t"
+        "s_mov_b32 %[kP], 9\n\t"
+        "s_branch 7f\n\t"
+        "630:\n\t"
+        "s_mov_b32 %[kP], 10\n\t"
+        "s_branch 7f\n\t"
+        "640:\n\t"
+        "s_mov_b32 %[kP], 0\n\t"
+        "s_branch 19f\n\t"
+        "641:\n\t"
+        "s_mov_b32 %[kP], 1\n\t"
+        "s_branch 19f\n\t"
+        "642:\n\t"
+        "s_mov_b32 %[kP], 2\n\t"
+        "s_branch 19f\n\t"
+        "643:\n\t"
+        "s_mov_b32 %[kP], 3\n\t"
+        "s_branch 19f\n\t"
+        "664:\n\t"
+        "s_mov_b32 %[kP], 4\n\t"
+        "s_branch 97f\n\t"
+        "665:\n\t"
+        "s_mov_b32 %[kP], 5\n\t"
+        "s_branch 97f\n\t"
+        "666:\n\t"
+        "s_mov_b32 %[kP], 6\n\t"
+        "s_branch 97f\n\t"
+        "667:\n\t"
+        "s_mov_b32 %[kP], 7\n\t"
+        "s_branch 97f\n\t"
+        "668:\n\t"
+        "s_mov_b32 %[kP], 8\n\t"
+        "s_branch 97f\n\t"
+        "669:\n\t"
+        "s_mov_b32 %[kP], 9\n\t"
+        "s_branch 97f\n\t"
+        "670:\n\t"
+        "s_mov_b32 %[kP], 10\n\t"
+        "s_branch 97f\n\t"
+        "671:\n\t"
+        "s_mov_b32 %[kP], 11\n\t"
+        "s_branch 97f\n\t"
+        "684:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_mov_b32 %[kP], 4\n\t"
+        "s_branch 9f\n\t"
+        "685:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_mov_b32 %[kP], 5\n\t"
+        "s_branch 9f\n\t"
+        "686:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_mov_b32 %[kP], 6\n\t"
+        "s_branch 9f\n\t"
+        "687:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_mov_b32 %[kP], 7\n\t"
+        "s_branch 9f\n\t"
+        "688:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_mov_b32 %[kP], 8\n\t"
+        "s_branch 9f\n\t"
+        "689:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_mov_b32 %[kP], 9\n\t"
+        "s_branch 9f\n\t"
+        "690:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_mov_b32 %[kP], 10\n\t"
+        "s_branch 9f\n\t"
+        "691:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_mov_b32 %[kP], 11\n\t"
+        "s_branch 9f\n\t"
+        "704:\n\t"
+        "s_mov_b32 %[kP], 4\n\t"
+        "s_mov_b32 s87, 4\n\t"
+        "s_branch 36f\n\t"
+        "705:\n\t"
+        "s_mov_b32 %[kP], 5\n\t"
+        "s_mov_b32 s87, 5\n\t"
+        "s_branch 36f\n\t"
+        "706:\n\t"
+        "s_mov_b32 %[kP], 6\n\t"
+        "s_mov_b32 s87, 6\n\t"
+        "s_branch 36f\n\t"
+        "707:\n\t"
+        "s_mov_b32 %[kP], 7\n\t"
+        "s_mov_b32 s87, 7\n\t"
+        "s_branch 36f\n\t"
+        "708:\n\t"
+        "s_mov_b32 %[kP], 8\n\t"
+        "s_mov_b32 s87, 8\n\t"
+        "s_branch 36f\n\t"
+        "709:\n\t"
+        "s_mov_b32 %[kP], 9\n\t"
+        "s_mov_b32 s87, 9\n\t"
+        "s_branch 36f\n\t"
+        "710:\n\t"
+        "s_mov_b32 %[kP], 10\n\t"
+        "s_mov_b32 s87, 10\n\t"
+        "s_branch 36f\n\t"
+        "711:\n\t"
+        "s_mov_b32 %[kP], 11\n\t"
+        "s_mov_b32 s87, 11\n\t"
+        "s_branch 36f\n\t"
+        "720:\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_branch 9f\n\t"
+        "721:\n\t"
+        "s_sub_u32 %[kP], %[kP], 1\n\t"
+        "s_branch 7f\n\t"
+        "722:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
-        "47:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
-        "s_branch 9f\n\t"
-        "67:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
-        "s_branch 9f\n\t"
-        "77:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
-        "s_branch 9f\n\t"
+        "723:\n\t"
+        "s_mov_b32 s87, %[kP]\n\t"
         "36:\n\t"
+        /* ---- run prefix with one-bits (s87 = k_P) ---- */
         "s_not_b64 s[80:81], s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_mov_b32 s82, 0\n\t"
-        "s_add_u32 s87, %[kP], 4\n\t"
         "31:\n\t"
         "s_cmp_gt_u32 s84, 6\n\t"
         "s_cbranch_scc1 9f\n\t"
@@ -683,7 +1271,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_sub_u32 %[kP], s87, 5\n\t"
+        "s_sub_u32 %[kP], s87, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
@@ -691,10 +1279,8 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "v_writelane_b32 %[aF], 1, m0\n\t"
         "39:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_cbranch_scc1 8f\n\t"
-        "s_cmp_ge_i32 %[kP], 0\n\t"
-        "s_cbranch_scc0 10b\n\t"
-        "s_branch 30b\n\t"
+        "s_cbranch_scc1 9f\n\t"
+        "s_branch 91b\n\t"
         "38:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 98f\n\t"
@@ -702,7 +1288,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s91, %[kRP], s84\n\t"
-        "s_sub_u32 %[kP], s87, 5\n\t"
+        "s_sub_u32 %[kP], s87, 1\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
@@ -730,26 +1316,9 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "98:\n\t"
         "s_add_u32 %[nbx], %[nbx], s86\n\t"
         "s_branch 9f\n\t"
-        "85:\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_branch 9f\n\t"
-        "86:\n\t"
-        "s_max_i32 %[kP], %[kP], -4\n\t"
-        "s_add_u32 %[kP], %[kP], 3\n\t"
-        "s_branch 9f\n\t"
-        "87:\n\t"
-        "s_sub_u32 %[kP], %[kP], 1\n\t"
-        "s_branch 7f\n\t"
-        "88:\n\t"
-        "s_max_i32 %[kP], %[kP], -4\n\t"
-        "s_add_u32 %[kP], %[kP], 3\n\t"
         "7:\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_branch 9f\n\t"
-        "8:\n\t"
         "9:\n\t"
-        "s_max_i32 %[kP], %[kP], -4\n\t"
-        "s_add_u32 %[kP], %[kP], 4\n\t"
         "s_mov_b32 %[neg], m0\n\t"
         "s_mov_b64 %[win], s[94:95]\n\t"
 

@@ -27,6 +27,11 @@ def _cases(oracle, seed, streams_per_kind, states_per_stream):
             bnd = R.decode_boundaries(bits, L)
             for _ in range(states_per_stream):
                 pos, kP, kRP, _n = bnd[int(rng.integers(0, len(bnd)))]
+                if rng.random() < 1 / 3:
+                    # the decoder's state at this bit replaced by any other:
+                    # the bits after it are then read as that state reads
+                    # them, which enters every k_P state of the loop
+                    kP, kRP = int(rng.integers(0, 16)), int(rng.integers(0, 101))
                 nb = int(rng.integers(R.NB_MIN, R.NB_MAX + 1))
                 wpos = int(rng.integers(0, 1 << 20))
                 lim = int(rng.integers(1, 65))
@@ -34,7 +39,7 @@ def _cases(oracle, seed, streams_per_kind, states_per_stream):
                 yield kind, bits, pos, R.make_state(bits, pos, kP, kRP, rng, nb, wpos, lim, nrel)
 
 
-def _check(oracle, lines, seed=5, streams_per_kind=3, states_per_stream=40):
+def _check(oracle, lines, seed=5, streams_per_kind=3, states_per_stream=160):
     loop = R.Loop(lines)
     visited = set()
     fails = []
@@ -76,21 +81,24 @@ def test_scalar_loop_matches_decoder(oracle):
 
 
 @pytest.mark.parametrize("old,new", [
-    ("s_add_u32 %[kP], %[kP], 3", "s_add_u32 %[kP], %[kP], 2"),      # zero symbol: k_P step
+    ("s_branch 104b", "s_branch 105b"),                              # zero symbol at k_P 1: k_P 4 next
+    ("s_cbranch_scc1 604f", "s_cbranch_scc1 605f"),                  # chunk end: k_P after the symbol
+    ("s_addc_u32 s86, s84, 2", "s_addc_u32 s86, s84, 1"),            # run code: k + 1 bits before it
     ("s_sub_u32 s88, 63, s85", "s_sub_u32 s88, 62, s85"),            # zero test over k_R + 1 bits
     ("v_writelane_b32 %[aF], 1, m0", "v_writelane_b32 %[aF], 0, m0"),  # run symbol stored as u, not u + 1
     ("v_writelane_b32 %[aK], s85, m0", "v_writelane_b32 %[aK], s84, m0"),  # remainder width k_R
     ("v_writelane_b32 %[aT], s95, m0", "v_writelane_b32 %[aT], s94, m0"),  # code bits: the window's top
     ("s_cmp_lg_u32 s84, 0", "s_cmp_eq_u32 s84, 0"),                  # k_RP step: p != 0
     ("s_max_i32 %[kRP], s91, 0", "s_mov_b32 %[kRP], s91"),           # run path: k_RP floor dropped
-    ("s_lshr_b32 s85, %[kP], 2", "s_lshr_b32 s85, %[kP], 1"),        # run exponent k = k_P / 4
+    ("s_lshr_b64 s[90:91], s[94:95], 62", "s_lshr_b64 s[90:91], s[94:95], 61"),  # k = 1: one length bit
+    ("s_lshr_b32 s85, %[kP], 2", "s_lshr_b32 s85, %[kP], 1"),        # k_P >= 12: k = k_P / 4
 ])
 def test_planted_loop_mistakes_are_caught(oracle, old, new):
     """A mistake planted in the loop text (on the CPU) fails the check."""
     lines = R.extract_loop()
     i = next(j for j, s in enumerate(lines) if s.strip() == old)
     lines = lines[:i] + [new] + lines[i + 1:]
-    fails, _, _ = _check(oracle, lines, streams_per_kind=1, states_per_stream=20)
+    fails, _, _ = _check(oracle, lines, streams_per_kind=1, states_per_stream=60)
     assert fails, f"planted '{new}' went unnoticed"
 
 
