@@ -184,12 +184,17 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * jumps back to S0a.  A zero k = 0 symbol (u = 0) leaves the line for its
  * state's zero block (560-563), which shifts the window, lowers k_RP and
  * jumps to the block of k_P + 3 (S3, or R4..R6: run mode); its lanes decode
- * to 0.  A run prefix with one-bits (36) ends with k_P in its register and
- * enters its block through the dispatch (91) that also starts the loop.
- * Every exit first writes k_P (the stubs 600-723).  Per symbol this saves
- * the k_P step and its mode test, and on the run path the exponent and two
- * shift counts: 14.2 -> 12.7 scalar instructions per symbol on c1 colour
- * streams.  k_RP's step (p ? p - 1 : -2,
+ * to 0.  A run prefix of exactly one one-bit (82 % of the prefixes with
+ * one-bits on c1 streams) has its own block per run state (734-741): the
+ * one-bit stands for 2^k zeros, k_P + 2 fixes the width k' = (k_P + 2) / 4
+ * of the length that follows at a known place (one s_bfe), and the event
+ * ends in state k_P + 1, a known block.  Longer prefixes take the general
+ * path (36), which ends with k_P in its register and enters its block
+ * through the dispatch (91) that also starts the loop.  Every exit first
+ * writes k_P (the stubs 600-723).  Per symbol this saves the k_P step and
+ * its mode test, on the run path the exponent and two shift counts, and
+ * most of the general prefix path: 14.2 -> 11.5 scalar instructions per
+ * symbol on c1 colour streams.  k_RP's step (p ? p - 1 : -2,
  * clamped to [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the
  * carry-in of s_addc from the SCC of s_cmp_lg p, 0.  The clamp to 128 never binds here: a code
  * is decoded only when p + 1 + k_R <= 32 (run paths: p + k_R <= 29), and
@@ -292,7 +297,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "111:\n\t"
         /* ---- R11: k_P = 11, run entry without one-bits ---- */
         "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 711f\n\t"
+        "s_cbranch_scc1 741f\n\t"
         "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 691f\n\t"
@@ -318,7 +323,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "110:\n\t"
         /* ---- R10: k_P = 10, run entry without one-bits ---- */
         "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 710f\n\t"
+        "s_cbranch_scc1 740f\n\t"
         "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 690f\n\t"
@@ -344,7 +349,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "109:\n\t"
         /* ---- R9: k_P = 9, run entry without one-bits ---- */
         "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 709f\n\t"
+        "s_cbranch_scc1 739f\n\t"
         "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 689f\n\t"
@@ -370,7 +375,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "108:\n\t"
         /* ---- R8: k_P = 8, run entry without one-bits ---- */
         "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 708f\n\t"
+        "s_cbranch_scc1 738f\n\t"
         "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 688f\n\t"
@@ -396,7 +401,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "107:\n\t"
         /* ---- R7: k_P = 7, run entry without one-bits ---- */
         "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 707f\n\t"
+        "s_cbranch_scc1 737f\n\t"
         "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 687f\n\t"
@@ -422,7 +427,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "106:\n\t"
         /* ---- R6: k_P = 6, run entry without one-bits ---- */
         "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 706f\n\t"
+        "s_cbranch_scc1 736f\n\t"
         "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 686f\n\t"
@@ -448,7 +453,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "105:\n\t"
         /* ---- R5: k_P = 5, run entry without one-bits ---- */
         "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 705f\n\t"
+        "s_cbranch_scc1 735f\n\t"
         "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 685f\n\t"
@@ -474,7 +479,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "104:\n\t"
         /* ---- R4: k_P = 4, run entry without one-bits ---- */
         "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 704f\n\t"
+        "s_cbranch_scc1 734f\n\t"
         "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 684f\n\t"
@@ -670,6 +675,223 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 606f\n\t"
         "s_branch 106b\n\t"
+        "734:\n\t"
+        "s_bitcmp1_b64 s[94:95], 62\n\t"
+        "s_cbranch_scc1 704f\n\t"
+        "s_bfe_u64 s[90:91], s[94:95], 0x1003d\n\t"
+        "s_add_u32 s90, s90, 2\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 684f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 3\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 1134f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "934:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 605f\n\t"
+        "s_branch 105b\n\t"
+        "735:\n\t"
+        "s_bitcmp1_b64 s[94:95], 62\n\t"
+        "s_cbranch_scc1 705f\n\t"
+        "s_bfe_u64 s[90:91], s[94:95], 0x1003d\n\t"
+        "s_add_u32 s90, s90, 2\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 685f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 3\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 1135f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "935:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 606f\n\t"
+        "s_branch 106b\n\t"
+        "736:\n\t"
+        "s_bitcmp1_b64 s[94:95], 62\n\t"
+        "s_cbranch_scc1 706f\n\t"
+        "s_bfe_u64 s[90:91], s[94:95], 0x2003c\n\t"
+        "s_add_u32 s90, s90, 2\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 686f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 4\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 4\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 1136f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "936:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 607f\n\t"
+        "s_branch 107b\n\t"
+        "737:\n\t"
+        "s_bitcmp1_b64 s[94:95], 62\n\t"
+        "s_cbranch_scc1 707f\n\t"
+        "s_bfe_u64 s[90:91], s[94:95], 0x2003c\n\t"
+        "s_add_u32 s90, s90, 2\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 687f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 4\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 4\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 1137f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "937:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 608f\n\t"
+        "s_branch 108b\n\t"
+        "738:\n\t"
+        "s_bitcmp1_b64 s[94:95], 62\n\t"
+        "s_cbranch_scc1 708f\n\t"
+        "s_bfe_u64 s[90:91], s[94:95], 0x2003c\n\t"
+        "s_add_u32 s90, s90, 4\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 688f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 4\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 4\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 1138f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "938:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 609f\n\t"
+        "s_branch 109b\n\t"
+        "739:\n\t"
+        "s_bitcmp1_b64 s[94:95], 62\n\t"
+        "s_cbranch_scc1 709f\n\t"
+        "s_bfe_u64 s[90:91], s[94:95], 0x2003c\n\t"
+        "s_add_u32 s90, s90, 4\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 689f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 4\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 4\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 1139f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "939:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 610f\n\t"
+        "s_branch 110b\n\t"
+        "740:\n\t"
+        "s_bitcmp1_b64 s[94:95], 62\n\t"
+        "s_cbranch_scc1 710f\n\t"
+        "s_bfe_u64 s[90:91], s[94:95], 0x3003b\n\t"
+        "s_add_u32 s90, s90, 4\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 690f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 5\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 5\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 1140f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "940:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 611f\n\t"
+        "s_branch 111b\n\t"
+        "741:\n\t"
+        "s_bitcmp1_b64 s[94:95], 62\n\t"
+        "s_cbranch_scc1 711f\n\t"
+        "s_bfe_u64 s[90:91], s[94:95], 0x3003b\n\t"
+        "s_add_u32 s90, s90, 4\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_cbranch_scc1 691f\n\t"
+        "s_lshl_b64 s[92:93], s[94:95], 5\n\t"
+        "s_lshr_b32 s89, %[kRP], 2\n\t"
+        "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
+        "s_flbit_i32_b64 s84, s[80:81]\n\t"
+        "s_addc_u32 s86, s84, 5\n\t"
+        "s_add_u32 s86, s86, s89\n\t"
+        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "v_writelane_b32 %[aT], s93, m0\n\t"
+        "s_cbranch_scc1 1141f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "941:\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_cbranch_scc1 612f\n\t"
+        "s_mov_b32 %[kP], 12\n\t"
+        "s_branch 100b\n\t"
         "523:\n\t"
         "s_cmp_gt_u32 s87, 32\n\t"
         "s_cbranch_scc1 643f\n\t"
@@ -1033,6 +1255,166 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 304b\n\t"
+        "1134:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 664f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 625f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 934b\n\t"
+        "1135:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 665f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 626f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 935b\n\t"
+        "1136:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 666f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 627f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 936b\n\t"
+        "1137:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 667f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 628f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 937b\n\t"
+        "1138:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 668f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 629f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 938b\n\t"
+        "1139:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 669f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 630f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 939b\n\t"
+        "1140:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 670f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 631f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 940b\n\t"
+        "1141:\n\t"
+        "s_cmp_gt_u32 s86, 32\n\t"
+        "s_cbranch_scc1 671f\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
+        "v_writelane_b32 %[aK], s89, m0\n\t"
+        "s_cmp_lg_u32 s84, 0\n\t"
+        "v_writelane_b32 %[aF], 1, m0\n\t"
+        "s_addc_u32 s91, %[kRP], s84\n\t"
+        "s_sub_i32 s91, s91, 2\n\t"
+        "s_max_i32 %[kRP], s91, 0\n\t"
+        "s_add_u32 s88, %[wpos], 1\n\t"
+        "s_and_b32 s88, s88, 63\n\t"
+        "s_cbranch_scc0 632f\n\t"
+        "s_sub_u32 s89, -1, %[nbx]\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_add_i32 %[nbx], %[nbx], 32\n\t"
+        "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "v_readlane_b32 s96, %[cur], s88\n\t"
+        "s_branch 941b\n\t"
         "500:\n\t"
         "s_cmp_gt_u32 s86, 32\n\t"
         "s_cbranch_scc1 97f\n\t"
@@ -1086,6 +1468,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "610:\n\t"
         "s_mov_b32 %[kP], 10\n\t"
         "s_branch 9f\n\t"
+        "611:\n\t"
+        "s_mov_b32 %[kP], 11\n\t"
+        "s_branch 9f\n\t"
+        "612:\n\t"
+        "s_mov_b32 %[kP], 12\n\t"
+        "s_branch 9f\n\t"
         "620:\n\t"
         "s_mov_b32 %[kP], 0\n\t"
         "s_branch 7f\n\t"
@@ -1118,6 +1506,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_branch 7f\n\t"
         "630:\n\t"
         "s_mov_b32 %[kP], 10\n\t"
+        "s_branch 7f\n\t"
+        "631:\n\t"
+        "s_mov_b32 %[kP], 11\n\t"
+        "s_branch 7f\n\t"
+        "632:\n\t"
+        "s_mov_b32 %[kP], 12\n\t"
         "s_branch 7f\n\t"
         "640:\n\t"
         "s_mov_b32 %[kP], 0\n\t"

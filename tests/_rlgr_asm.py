@@ -176,6 +176,13 @@ class Loop:
                 v = (rd(a[1]) | ~rd(a[2])) & M64
                 wr(a[0], v)
                 scc = int(v != 0)
+            elif op == "s_bfe_u64":
+                # field at bit S1[5:0], S1[22:16] bits wide
+                f = rd(a[2])
+                off, width = f & 63, (f >> 16) & 127
+                v = (rd(a[1]) >> off) & ((1 << width) - 1) & M64
+                wr(a[0], v)
+                scc = int(v != 0)
             elif op == "s_bitcmp1_b64":
                 scc = (rd(a[0]) >> (rd(a[1]) & 63)) & 1
             elif op.startswith("s_cmp_"):

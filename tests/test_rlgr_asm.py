@@ -34,12 +34,15 @@ def _cases(oracle, seed, streams_per_kind, states_per_stream):
                     kP, kRP = int(rng.integers(0, 16)), int(rng.integers(0, 101))
                 nb = int(rng.integers(R.NB_MIN, R.NB_MAX + 1))
                 wpos = int(rng.integers(0, 1 << 20))
+                if rng.random() < 1 / 4:
+                    # a refill within the next few dwords rotates the chunk
+                    wpos = (wpos & ~63) | int(rng.integers(60, 64))
                 lim = int(rng.integers(1, 65))
                 nrel = int(rng.integers(0, lim))
                 yield kind, bits, pos, R.make_state(bits, pos, kP, kRP, rng, nb, wpos, lim, nrel)
 
 
-def _check(oracle, lines, seed=5, streams_per_kind=3, states_per_stream=160):
+def _check(oracle, lines, seed=5, streams_per_kind=3, states_per_stream=320):
     loop = R.Loop(lines)
     visited = set()
     fails = []
@@ -92,6 +95,8 @@ def test_scalar_loop_matches_decoder(oracle):
     ("s_max_i32 %[kRP], s91, 0", "s_mov_b32 %[kRP], s91"),           # run path: k_RP floor dropped
     ("s_lshr_b64 s[90:91], s[94:95], 62", "s_lshr_b64 s[90:91], s[94:95], 61"),  # k = 1: one length bit
     ("s_lshr_b32 s85, %[kP], 2", "s_lshr_b32 s85, %[kP], 1"),        # k_P >= 12: k = k_P / 4
+    ("s_add_u32 s90, s90, 2", "s_add_u32 s90, s90, 1"),              # one prefix one-bit: 2^k zeros
+    ("s_bitcmp1_b64 s[94:95], 62", "s_bitcmp1_b64 s[94:95], 61"),    # a second one-bit: general path
 ])
 def test_planted_loop_mistakes_are_caught(oracle, old, new):
     """A mistake planted in the loop text (on the CPU) fails the check."""
