@@ -142,7 +142,7 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
 /* Common-case RLGR symbols as a tight loop on the scalar unit
  * (entropy_codec.cc:207-283).  State in SGPRs: the 64-bit window `win`
  * (left aligned, zero below its nb valid bits; s[94:95] inside the loop, so
- * that its top dword s95 can be named) with nbx = nb - 33, the next dword
+ * that its top dword s95 can be named) with nn = 32 - nb, the next dword
  * `wpos` of the chunk held one dword per lane in `cur`, k_RP, k_P and
  * neg = nrel - lim (mod 2^32; its carry out marks the end of the 64-symbol
  * chunk).  Inside the loop neg lives in M0, the lane select v_writelane
@@ -167,11 +167,12 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  *    when the top k_R + 1 bits of the window are all zero; that test (one
  *    shift) chooses the zero block;
  *  - the window takes 32 more bits when fewer than 33 remain.  The code's
- *    length is subtracted from nbx as soon as it is known: no borrow (the
- *    common case) means the code lies in the window's first 32 bits and no
- *    refill follows, so that one test stands for both; a borrow leaves the
- *    line for a block that tests for a code over 32 bits (an exit, nbx
- *    restored), then finishes the symbol and refills.
+ *    length is added to nn as soon as it is known: no carry (the common
+ *    case) means the code lies in the window's first 32 bits and no refill
+ *    follows, so that one test stands for both; a carry leaves the line for
+ *    a block that tests for a code over 32 bits (an exit, nn restored), then
+ *    finishes the symbol and refills.  After a carry nn is 32 - nb of the
+ *    shorter window, the shift that places the next dword below it.
  * k_P is not held in a register: it is the position in the code.  One
  * block per k_P value decodes one symbol with k = k_P / 4 and the shift
  * counts of its run entry as constants: G (k_P >= 12, the only one that
@@ -184,7 +185,12 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * jumps back to S0a.  A zero k = 0 symbol (u = 0) leaves the line for its
  * state's zero block (560-563), which shifts the window, lowers k_RP and
  * jumps to the block of k_P + 3 (S3, or R4..R6: run mode); its lanes decode
- * to 0.  A run prefix of exactly one one-bit (82 % of the prefixes with
+ * to 0.  A run block reads the run length with an arithmetic shift of the
+ * window: when the prefix starts with a one-bit, that value is negative and
+ * its add to neg always carries, so the one branch for a zero run past the
+ * chunk end also catches prefixes with one-bits; the block it leads to
+ * (680-691) undoes the add and tells the two apart by the sign word s91.
+ * A run prefix of exactly one one-bit (82 % of the prefixes with
  * one-bits on c1 streams) has its own block per run state (734-741): the
  * one-bit stands for 2^k zeros, k_P + 2 fixes the width k' = (k_P + 2) / 4
  * of the length that follows at a known place (one s_bfe), and the event
@@ -193,7 +199,7 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * through the dispatch (91) that also starts the loop.  Every exit first
  * writes k_P (the stubs 600-723).  Per symbol this saves the k_P step and
  * its mode test, on the run path the exponent and two shift counts, and
- * most of the general prefix path: 14.2 -> 11.5 scalar instructions per
+ * most of the general prefix path: 14.2 -> 11.2 scalar instructions per
  * symbol on c1 colour streams.  k_RP's step (p ? p - 1 : -2,
  * clamped to [0, 128]) is kRP + p + [p != 0] - 2, the bracket taken as the
  * carry-in of s_addc from the SCC of s_cmp_lg p, 0.  The clamp to 128 never binds here: a code
@@ -203,10 +209,9 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * i.e. k_RP >= 4.  Only the zero symbols and the run paths (where u = 0 is a
  * valid symbol, 1) keep it.  A run entry whose prefix has no one-bits
  * (the common case) is the run block itself: its zero count is the k-bit
- * suffix alone and `neg` advances in place (undone on the out-of-line exits
- * 684-691, 722); prefixes with one-bits take the one general path, laid out
- * as entry (36), one-bits loop (31/35) and symbol (32) in a line; a one-bit
- * test of the window's top bit chooses between them.  The
+ * suffix alone and `neg` advances in place (undone in 680-691 and 722);
+ * prefixes of two or more one-bits take the one general path, laid out as
+ * entry (36), one-bits loop (31/35) and symbol (32) in a line.  The
  * leading-ones count of a Golomb code is s_flbit of s_orn2(1, window): that
  * value is never 0, so the count stays below 64 and s_orn2 sets SCC, which
  * s_flbit leaves for the s_addc that adds the code's terminating zero.  A
@@ -215,16 +220,16 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * line, one block per state copy (block + 400, 580-583, 38).
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window; every exit leaves the state of the last whole symbol (the
- * early nbx and m0 updates are undone on the way out: 19, 97, 98, 684-691
- * and 722).  The loop returns
+ * early nn and m0 updates are undone on the way out: 19, 97, 98, 680-691,
+ * 774-781 and 722; a chunk rotation also takes back the wpos step, 7).  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
  * of `cur`, a unary prefix of 32+ or a code longer than 32 bits, more than 6
  * one-bits in a run prefix or k > 24 after them, or a zero run reaching the
  * chunk end.  s80-s97 are scratch.  An SGPR written by a scalar
  * instruction and then read as a v_readlane / v_writelane lane select gets
- * 4 wait states (here at least six instructions on every path;
+ * 4 wait states (here at least four instructions on every path;
  * tests/test_rlgr_asm.py checks the distances on the loop text). */
-__device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint32_t& wpos, uint32_t& kRP,
+__device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nn, uint32_t& wpos, uint32_t& kRP,
                                                uint32_t& kP, uint32_t& neg, uint32_t cur, uint32_t pre, uint32_t& aT,
                                                uint32_t& aK, uint32_t& aF) {
     asm volatile(
@@ -278,7 +283,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 500f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -296,9 +301,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc0 100b\n\t"
         "111:\n\t"
         /* ---- R11: k_P = 11, run entry without one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 741f\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
+        "s_ashr_i64 s[90:91], s[94:95], 61\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 691f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
@@ -307,7 +310,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 3\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 511f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -322,9 +325,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 610f\n\t"
         "110:\n\t"
         /* ---- R10: k_P = 10, run entry without one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 740f\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
+        "s_ashr_i64 s[90:91], s[94:95], 61\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 690f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
@@ -333,7 +334,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 3\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 510f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -348,9 +349,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 609f\n\t"
         "109:\n\t"
         /* ---- R9: k_P = 9, run entry without one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 739f\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
+        "s_ashr_i64 s[90:91], s[94:95], 61\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 689f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
@@ -359,7 +358,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 3\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 509f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -374,9 +373,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 608f\n\t"
         "108:\n\t"
         /* ---- R8: k_P = 8, run entry without one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 738f\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], 61\n\t"
+        "s_ashr_i64 s[90:91], s[94:95], 61\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 688f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
@@ -385,7 +382,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 3\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 508f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -400,9 +397,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 607f\n\t"
         "107:\n\t"
         /* ---- R7: k_P = 7, run entry without one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 737f\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
+        "s_ashr_i64 s[90:91], s[94:95], 62\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 687f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 2\n\t"
@@ -411,7 +406,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 2\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 507f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -426,9 +421,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 606f\n\t"
         "106:\n\t"
         /* ---- R6: k_P = 6, run entry without one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 736f\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
+        "s_ashr_i64 s[90:91], s[94:95], 62\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 686f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 2\n\t"
@@ -437,7 +430,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 2\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 506f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -452,9 +445,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 605f\n\t"
         "105:\n\t"
         /* ---- R5: k_P = 5, run entry without one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 735f\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
+        "s_ashr_i64 s[90:91], s[94:95], 62\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 685f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 2\n\t"
@@ -463,7 +454,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 2\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 505f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -478,9 +469,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cbranch_scc1 604f\n\t"
         "104:\n\t"
         /* ---- R4: k_P = 4, run entry without one-bits ---- */
-        "s_bitcmp1_b64 s[94:95], 63\n\t"
-        "s_cbranch_scc1 734f\n\t"
-        "s_lshr_b64 s[90:91], s[94:95], 62\n\t"
+        "s_ashr_i64 s[90:91], s[94:95], 62\n\t"
         "s_add_u32 m0, m0, s90\n\t"
         "s_cbranch_scc1 684f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 2\n\t"
@@ -489,7 +478,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 2\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 504f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -508,7 +497,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_add_u32 %[nn], %[nn], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 523f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
@@ -528,7 +517,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_add_u32 %[nn], %[nn], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 522f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
@@ -548,7 +537,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_add_u32 %[nn], %[nn], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 521f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
@@ -568,7 +557,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_add_u32 %[nn], %[nn], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 520f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
@@ -588,7 +577,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_add_u32 %[nn], %[nn], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 530f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
@@ -608,7 +597,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_add_u32 %[nn], %[nn], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 540f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
@@ -628,7 +617,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_orn2_b64 s[80:81], 1, s[94:95]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s87, s84, s85\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s87\n\t"
+        "s_add_u32 %[nn], %[nn], s87\n\t"
         "v_writelane_b32 %[aT], s95, m0\n\t"
         "s_cbranch_scc1 550f\n\t"
         "s_sub_u32 s88, 63, s85\n\t"
@@ -675,20 +664,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 606f\n\t"
         "s_branch 106b\n\t"
+        "684:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_cmp_eq_u32 s91, 0\n\t"
+        "s_cbranch_scc1 754f\n\t"
         "734:\n\t"
         "s_bitcmp1_b64 s[94:95], 62\n\t"
         "s_cbranch_scc1 704f\n\t"
         "s_bfe_u64 s[90:91], s[94:95], 0x1003d\n\t"
         "s_add_u32 s90, s90, 2\n\t"
         "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 684f\n\t"
+        "s_cbranch_scc1 774f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 3\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 1134f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -702,20 +695,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 605f\n\t"
         "s_branch 105b\n\t"
+        "685:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_cmp_eq_u32 s91, 0\n\t"
+        "s_cbranch_scc1 755f\n\t"
         "735:\n\t"
         "s_bitcmp1_b64 s[94:95], 62\n\t"
         "s_cbranch_scc1 705f\n\t"
         "s_bfe_u64 s[90:91], s[94:95], 0x1003d\n\t"
         "s_add_u32 s90, s90, 2\n\t"
         "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 685f\n\t"
+        "s_cbranch_scc1 775f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 3\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 3\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 1135f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -729,20 +726,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 606f\n\t"
         "s_branch 106b\n\t"
+        "686:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_cmp_eq_u32 s91, 0\n\t"
+        "s_cbranch_scc1 756f\n\t"
         "736:\n\t"
         "s_bitcmp1_b64 s[94:95], 62\n\t"
         "s_cbranch_scc1 706f\n\t"
         "s_bfe_u64 s[90:91], s[94:95], 0x2003c\n\t"
         "s_add_u32 s90, s90, 2\n\t"
         "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 686f\n\t"
+        "s_cbranch_scc1 776f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 4\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 4\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 1136f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -756,20 +757,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 607f\n\t"
         "s_branch 107b\n\t"
+        "687:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_cmp_eq_u32 s91, 0\n\t"
+        "s_cbranch_scc1 757f\n\t"
         "737:\n\t"
         "s_bitcmp1_b64 s[94:95], 62\n\t"
         "s_cbranch_scc1 707f\n\t"
         "s_bfe_u64 s[90:91], s[94:95], 0x2003c\n\t"
         "s_add_u32 s90, s90, 2\n\t"
         "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 687f\n\t"
+        "s_cbranch_scc1 777f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 4\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 4\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 1137f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -783,20 +788,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 608f\n\t"
         "s_branch 108b\n\t"
+        "688:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_cmp_eq_u32 s91, 0\n\t"
+        "s_cbranch_scc1 758f\n\t"
         "738:\n\t"
         "s_bitcmp1_b64 s[94:95], 62\n\t"
         "s_cbranch_scc1 708f\n\t"
         "s_bfe_u64 s[90:91], s[94:95], 0x2003c\n\t"
         "s_add_u32 s90, s90, 4\n\t"
         "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 688f\n\t"
+        "s_cbranch_scc1 778f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 4\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 4\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 1138f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -810,20 +819,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 609f\n\t"
         "s_branch 109b\n\t"
+        "689:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_cmp_eq_u32 s91, 0\n\t"
+        "s_cbranch_scc1 759f\n\t"
         "739:\n\t"
         "s_bitcmp1_b64 s[94:95], 62\n\t"
         "s_cbranch_scc1 709f\n\t"
         "s_bfe_u64 s[90:91], s[94:95], 0x2003c\n\t"
         "s_add_u32 s90, s90, 4\n\t"
         "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 689f\n\t"
+        "s_cbranch_scc1 779f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 4\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 4\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 1139f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -837,20 +850,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 610f\n\t"
         "s_branch 110b\n\t"
+        "690:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_cmp_eq_u32 s91, 0\n\t"
+        "s_cbranch_scc1 760f\n\t"
         "740:\n\t"
         "s_bitcmp1_b64 s[94:95], 62\n\t"
         "s_cbranch_scc1 710f\n\t"
         "s_bfe_u64 s[90:91], s[94:95], 0x3003b\n\t"
         "s_add_u32 s90, s90, 4\n\t"
         "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 690f\n\t"
+        "s_cbranch_scc1 780f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 5\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 5\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 1140f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -864,20 +881,24 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 m0, m0, 1\n\t"
         "s_cbranch_scc1 611f\n\t"
         "s_branch 111b\n\t"
+        "691:\n\t"
+        "s_sub_u32 m0, m0, s90\n\t"
+        "s_cmp_eq_u32 s91, 0\n\t"
+        "s_cbranch_scc1 761f\n\t"
         "741:\n\t"
         "s_bitcmp1_b64 s[94:95], 62\n\t"
         "s_cbranch_scc1 711f\n\t"
         "s_bfe_u64 s[90:91], s[94:95], 0x3003b\n\t"
         "s_add_u32 s90, s90, 4\n\t"
         "s_add_u32 m0, m0, s90\n\t"
-        "s_cbranch_scc1 691f\n\t"
+        "s_cbranch_scc1 781f\n\t"
         "s_lshl_b64 s[92:93], s[94:95], 5\n\t"
         "s_lshr_b32 s89, %[kRP], 2\n\t"
         "s_orn2_b64 s[80:81], 1, s[92:93]\n\t"
         "s_flbit_i32_b64 s84, s[80:81]\n\t"
         "s_addc_u32 s86, s84, 5\n\t"
         "s_add_u32 s86, s86, s89\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "s_cbranch_scc1 1141f\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -903,14 +924,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 622f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 622f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 323b\n\t"
         "522:\n\t"
@@ -924,14 +943,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 621f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 621f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 322b\n\t"
         "521:\n\t"
@@ -945,14 +962,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 620f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 620f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 321b\n\t"
         "520:\n\t"
@@ -966,14 +981,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 620f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 620f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 320b\n\t"
         "530:\n\t"
@@ -987,14 +1000,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 620f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 620f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 330b\n\t"
         "540:\n\t"
@@ -1008,14 +1019,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 620f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 620f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 340b\n\t"
         "550:\n\t"
@@ -1029,70 +1038,60 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_cmp_lg_u32 s84, 0\n\t"
         "s_addc_u32 s90, %[kRP], s84\n\t"
         "s_sub_i32 %[kRP], s90, 2\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 620f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 620f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 350b\n\t"
         "580:\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 623f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 623f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 570b\n\t"
         "581:\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 624f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 624f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 571b\n\t"
         "582:\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 625f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 625f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 572b\n\t"
         "583:\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s87\n\t"
         "s_sub_i32 s90, %[kRP], 2\n\t"
         "s_max_i32 %[kRP], s90, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 626f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 626f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 573b\n\t"
         "511:\n\t"
@@ -1105,14 +1104,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 630f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 630f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 311b\n\t"
         "510:\n\t"
@@ -1125,14 +1122,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 629f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 629f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 310b\n\t"
         "509:\n\t"
@@ -1145,14 +1140,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 628f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 628f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 309b\n\t"
         "508:\n\t"
@@ -1165,14 +1158,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 627f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 627f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 308b\n\t"
         "507:\n\t"
@@ -1185,14 +1176,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 626f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 626f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 307b\n\t"
         "506:\n\t"
@@ -1205,14 +1194,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 625f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 625f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 306b\n\t"
         "505:\n\t"
@@ -1225,14 +1212,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 624f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 624f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 305b\n\t"
         "504:\n\t"
@@ -1245,14 +1230,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 623f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 623f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 304b\n\t"
         "1134:\n\t"
@@ -1265,14 +1248,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 625f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 625f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 934b\n\t"
         "1135:\n\t"
@@ -1285,14 +1266,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 626f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 626f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 935b\n\t"
         "1136:\n\t"
@@ -1305,14 +1284,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 627f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 627f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 936b\n\t"
         "1137:\n\t"
@@ -1325,14 +1302,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 628f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 628f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 937b\n\t"
         "1138:\n\t"
@@ -1345,14 +1320,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 629f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 629f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 938b\n\t"
         "1139:\n\t"
@@ -1365,14 +1338,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 630f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 630f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 939b\n\t"
         "1140:\n\t"
@@ -1385,14 +1356,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 631f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 631f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 940b\n\t"
         "1141:\n\t"
@@ -1405,14 +1374,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 632f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 632f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 941b\n\t"
         "500:\n\t"
@@ -1425,14 +1392,12 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_addc_u32 s91, %[kRP], s84\n\t"
         "s_sub_i32 s91, s91, 2\n\t"
         "s_max_i32 %[kRP], s91, 0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 721f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 721f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 300b\n\t"
         "600:\n\t"
@@ -1549,35 +1514,59 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "671:\n\t"
         "s_mov_b32 %[kP], 11\n\t"
         "s_branch 97f\n\t"
-        "684:\n\t"
+        "754:\n\t"
+        "s_mov_b32 %[kP], 4\n\t"
+        "s_branch 9f\n\t"
+        "755:\n\t"
+        "s_mov_b32 %[kP], 5\n\t"
+        "s_branch 9f\n\t"
+        "756:\n\t"
+        "s_mov_b32 %[kP], 6\n\t"
+        "s_branch 9f\n\t"
+        "757:\n\t"
+        "s_mov_b32 %[kP], 7\n\t"
+        "s_branch 9f\n\t"
+        "758:\n\t"
+        "s_mov_b32 %[kP], 8\n\t"
+        "s_branch 9f\n\t"
+        "759:\n\t"
+        "s_mov_b32 %[kP], 9\n\t"
+        "s_branch 9f\n\t"
+        "760:\n\t"
+        "s_mov_b32 %[kP], 10\n\t"
+        "s_branch 9f\n\t"
+        "761:\n\t"
+        "s_mov_b32 %[kP], 11\n\t"
+        "s_branch 9f\n\t"
+        "774:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_mov_b32 %[kP], 4\n\t"
         "s_branch 9f\n\t"
-        "685:\n\t"
+        "775:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_mov_b32 %[kP], 5\n\t"
         "s_branch 9f\n\t"
-        "686:\n\t"
+        "776:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_mov_b32 %[kP], 6\n\t"
         "s_branch 9f\n\t"
-        "687:\n\t"
+        "777:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_mov_b32 %[kP], 7\n\t"
         "s_branch 9f\n\t"
-        "688:\n\t"
+        "778:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_mov_b32 %[kP], 8\n\t"
         "s_branch 9f\n\t"
-        "689:\n\t"
+        "779:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_mov_b32 %[kP], 9\n\t"
         "s_branch 9f\n\t"
-        "690:\n\t"
+        "780:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_mov_b32 %[kP], 10\n\t"
         "s_branch 9f\n\t"
-        "691:\n\t"
+        "781:\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_mov_b32 %[kP], 11\n\t"
         "s_branch 9f\n\t"
@@ -1659,7 +1648,7 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "s_add_u32 s86, s86, s84\n\t"
         "s_add_u32 s86, s86, s89\n\t"
         "s_add_u32 s86, s86, 1\n\t"
-        "s_sub_u32 %[nbx], %[nbx], s86\n\t"
+        "s_add_u32 %[nn], %[nn], s86\n\t"
         "s_cbranch_scc1 38f\n\t"
         "s_mov_b32 m0, s83\n\t"
         "s_lshl_b64 s[94:95], s[94:95], s86\n\t"
@@ -1688,35 +1677,34 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nbx, uint3
         "v_writelane_b32 %[aT], s93, m0\n\t"
         "v_writelane_b32 %[aK], s89, m0\n\t"
         "v_writelane_b32 %[aF], 1, m0\n\t"
-        "s_add_u32 s88, %[wpos], 1\n\t"
-        "s_and_b32 s88, s88, 63\n\t"
-        "s_cbranch_scc0 7f\n\t"
-        "s_sub_u32 s89, -1, %[nbx]\n\t"
-        "s_lshl_b64 s[90:91], s[96:97], s89\n\t"
-        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
-        "s_add_i32 %[nbx], %[nbx], 32\n\t"
         "s_add_u32 %[wpos], %[wpos], 1\n\t"
+        "s_and_b32 s88, %[wpos], 63\n\t"
+        "s_cbranch_scc0 7f\n\t"
+        "s_lshl_b64 s[90:91], s[96:97], %[nn]\n\t"
+        "s_or_b64 s[94:95], s[94:95], s[90:91]\n\t"
+        "s_sub_u32 %[nn], %[nn], 32\n\t"
         "v_readlane_b32 s96, %[cur], s88\n\t"
         "s_branch 39b\n\t"
         "19:\n\t"
         "v_writelane_b32 %[aT], 0, m0\n\t"
-        "s_add_u32 %[nbx], %[nbx], s87\n\t"
+        "s_sub_u32 %[nn], %[nn], s87\n\t"
         "s_branch 9f\n\t"
         "97:\n\t"
         "v_writelane_b32 %[aT], 0, m0\n\t"
-        "s_add_u32 %[nbx], %[nbx], s86\n\t"
+        "s_sub_u32 %[nn], %[nn], s86\n\t"
         "s_sub_u32 m0, m0, s90\n\t"
         "s_branch 9f\n\t"
         "98:\n\t"
-        "s_add_u32 %[nbx], %[nbx], s86\n\t"
+        "s_sub_u32 %[nn], %[nn], s86\n\t"
         "s_branch 9f\n\t"
         "7:\n\t"
+        "s_sub_u32 %[wpos], %[wpos], 1\n\t"
         "s_add_u32 m0, m0, 1\n\t"
         "9:\n\t"
         "s_mov_b32 %[neg], m0\n\t"
         "s_mov_b64 %[win], s[94:95]\n\t"
 
-        : [win] "+s"(win), [nbx] "+s"(nbx), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [neg] "+s"(neg),
+        : [win] "+s"(win), [nn] "+s"(nn), [wpos] "+s"(wpos), [kRP] "+s"(kRP), [kP] "+s"(kP), [neg] "+s"(neg),
           [aT] "+v"(aT), [aK] "+v"(aK), [aF] "+v"(aF)
         : [cur] "v"(cur), [pre] "s"(pre)
         : "scc", "m0", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92",
@@ -1789,15 +1777,15 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
             kRP = __builtin_amdgcn_readfirstlane(kRP);
             kP = __builtin_amdgcn_readfirstlane(kP);
             uint32_t neg = __builtin_amdgcn_readfirstlane(nrel - lim);
-            int32_t nbx = __builtin_amdgcn_readfirstlane(br.nb - 33);
+            int32_t nn = __builtin_amdgcn_readfirstlane(32 - br.nb);
             /* the loop takes the next dword (the window's next refill) in
              * an SGPR and reads the one after it during each refill */
             br_sync(br);
             const uint32_t pre =
                 static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(br.cur), static_cast<int>(br.wpos & 63u)));
-            rlgr_fast_run(br.win, nbx, br.wpos, kRP, kP, neg, br.cur, pre, aT, aK, aF);
+            rlgr_fast_run(br.win, nn, br.wpos, kRP, kP, neg, br.cur, pre, aT, aK, aF);
             nrel = neg + lim;
-            br.nb = nbx + 33;
+            br.nb = 32 - nn;
         }
         if (nrel >= lim) { /* chunk of 64 symbols complete */
             store_chunk(acc_base);

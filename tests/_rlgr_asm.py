@@ -8,7 +8,7 @@ decoder independently (entropy_codec.cc:207-283: adaptive Golomb-Rice with
 run mode, L = 4, k_P from 0, k_RP from 8), so that a test can check every
 state the loop leaves against the decoder's definition.
 
-Register model: the operands %[win] (64-bit), %[nbx], %[wpos], %[kRP], %[kP],
+Register model: the operands %[win] (64-bit), %[nn], %[wpos], %[kRP], %[kP],
 %[neg] (32-bit scalars); %[aT], %[aK], %[aF] (per-lane accumulators,
 read/write), %[cur] (per-lane input), %[pre] (its dword wpos); s80-s97 and
 m0 scratch.  Symbol i of a
@@ -22,9 +22,9 @@ from pathlib import Path
 
 M32 = (1 << 32) - 1
 M64 = (1 << 64) - 1
-# The loop keeps nbx = nb - NB_BIAS for a window of nb valid bits and is
-# entered with NB_MIN <= nb <= NB_MAX.
-NB_BIAS, NB_MIN, NB_MAX = 33, 33, 64
+# The loop keeps nn = NN_BIAS - nb (mod 2^32) for a window of nb valid bits
+# and is entered with NB_MIN <= nb <= NB_MAX.
+NN_BIAS, NB_MIN, NB_MAX = 32, 33, 64
 HIP_SOURCE = Path(__file__).resolve().parents[1] / "patchvvc-plus_amd" / "csrc" / "pvvc_decode.hip"
 
 
@@ -176,6 +176,11 @@ class Loop:
                 v = (rd(a[1]) | ~rd(a[2])) & M64
                 wr(a[0], v)
                 scc = int(v != 0)
+            elif op == "s_ashr_i64":
+                x, y = rd(a[1]) & M64, rd(a[2]) & 63
+                v = ((x - (1 << 64) if x >> 63 else x) >> y) & M64
+                wr(a[0], v)
+                scc = int(v != 0)
             elif op == "s_bfe_u64":
                 # field at bit S1[5:0], S1[22:16] bits wide
                 f = rd(a[2])
@@ -317,7 +322,7 @@ def make_state(bits: list[int], pos: int, kP: int, kRP: int, rng, nb: int, wpos:
         acc[lane] = aT[lane] = 0
         aK[lane] = (7 * lane + nrel) % 40
     return {
-        "win": win, "nbx": (nb - NB_BIAS) & M32, "wpos": wpos & M32, "kRP": kRP, "kP": kP,
+        "win": win, "nn": (NN_BIAS - nb) & M32, "wpos": wpos & M32, "kRP": kRP, "kP": kP,
         "neg": (nrel - lim) & M32, "aT": aT, "aK": aK, "aF": aF, "acc": acc, "cur": cur, "pre": cur[l0],
         "lim": lim,
     }
@@ -345,7 +350,7 @@ def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> st
     whole number of events from st0; otherwise a description."""
     def s32(v):
         return v - (1 << 32) if v >> 31 else v
-    nb0, nb1 = s32(st0["nbx"]) + NB_BIAS, s32(st1["nbx"]) + NB_BIAS
+    nb0, nb1 = NN_BIAS - s32(st0["nn"]), NN_BIAS - s32(st1["nn"])
     dw = (st1["wpos"] - st0["wpos"]) & M32
     consumed = nb0 + 32 * dw - nb1
     lim = st0["lim"]

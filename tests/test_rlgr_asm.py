@@ -93,7 +93,8 @@ def test_scalar_loop_matches_decoder(oracle):
     ("v_writelane_b32 %[aT], s95, m0", "v_writelane_b32 %[aT], s94, m0"),  # code bits: the window's top
     ("s_cmp_lg_u32 s84, 0", "s_cmp_eq_u32 s84, 0"),                  # k_RP step: p != 0
     ("s_max_i32 %[kRP], s91, 0", "s_mov_b32 %[kRP], s91"),           # run path: k_RP floor dropped
-    ("s_lshr_b64 s[90:91], s[94:95], 62", "s_lshr_b64 s[90:91], s[94:95], 61"),  # k = 1: one length bit
+    ("s_ashr_i64 s[90:91], s[94:95], 62", "s_ashr_i64 s[90:91], s[94:95], 61"),  # k = 1: one length bit
+    ("s_sub_u32 m0, m0, s90", "s_sub_u32 m0, m0, 1"),                # run length added before the prefix test
     ("s_lshr_b32 s85, %[kP], 2", "s_lshr_b32 s85, %[kP], 1"),        # k_P >= 12: k = k_P / 4
     ("s_add_u32 s90, s90, 2", "s_add_u32 s90, s90, 1"),              # one prefix one-bit: 2^k zeros
     ("s_bitcmp1_b64 s[94:95], 62", "s_bitcmp1_b64 s[94:95], 61"),    # a second one-bit: general path
@@ -127,8 +128,8 @@ def test_wait_state_check_catches_planted_violations():
     i = lines.index("v_readlane_b32 s96, %[cur], s88")
     close = lines[:i - 1] + ["s_and_b32 s88, s88, 63"] + lines[i - 1:]
     assert any("lane select s88" in b for b in R.hazard_report(close))
-    # the refill computes its lane (s_and) six instructions before the read
-    j = max(k for k in range(i) if lines[k] == "s_and_b32 s88, s88, 63")
+    # the refill computes its lane (s_and) four instructions before the read
+    j = max(k for k in range(i) if lines[k] == "s_and_b32 s88, %[wpos], 63")
     early = lines[:j + 1] + [lines[i]] + lines[j + 1:i] + lines[i + 1:]
     assert any("lane select s88" in b for b in R.hazard_report(early))
     # an m0 write right before a v_writelane
