@@ -1778,12 +1778,24 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
             kP = __builtin_amdgcn_readfirstlane(kP);
             uint32_t neg = __builtin_amdgcn_readfirstlane(nrel - lim);
             int32_t nn = __builtin_amdgcn_readfirstlane(32 - br.nb);
-            /* the loop takes the next dword (the window's next refill) in
-             * an SGPR and reads the one after it during each refill */
-            br_sync(br);
-            const uint32_t pre =
-                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(br.cur), static_cast<int>(br.wpos & 63u)));
-            rlgr_fast_run(br.win, nn, br.wpos, kRP, kP, neg, br.cur, pre, aT, aK, aF);
+            for (;;) {
+                /* the loop takes the next dword (the window's next refill)
+                 * in an SGPR and reads the one after it during each refill */
+                br_sync(br);
+                const uint32_t pre = static_cast<uint32_t>(
+                    __builtin_amdgcn_readlane(static_cast<int>(br.cur), static_cast<int>(br.wpos & 63u)));
+                rlgr_fast_run(br.win, nn, br.wpos, kRP, kP, neg, br.cur, pre, aT, aK, aF);
+                /* full chunks back to back: the chunk is complete (neg = 0),
+                 * the window holds 33+ bits (nn < 0) and the next chunk is a
+                 * full one too, so every lane stores and lim stays 64 */
+                if (neg != 0u || nn >= 0 || total - acc_base < 128u) break;
+                const int32_t s = unsign2sign(lane_symbol(aT, aK, aF));
+                coef[acc_base + lane] = static_cast<float>(static_cast<double>(s) * static_cast<double>(qp));
+                aT = 0;
+                aF = 0;
+                acc_base += 64u;
+                neg = 0u - 64u;
+            }
             nrel = neg + lim;
             br.nb = 32 - nn;
         }
