@@ -2513,15 +2513,25 @@ __global__ __launch_bounds__(64 * WAVES) void persistent_kernel(const ItemDev* _
     __shared__ LevelMeta meta;
     __shared__ uint32_t next;
     uint8_t* ws = slab + static_cast<uint64_t>(blockIdx.x) * slab_bytes;
-    /* a workgroup's first entries are its own (blockIdx.x * chunk), so every
-     * workgroup starts decoding without an atomic round trip; the shared
-     * counter hands out the entries after the first grid * chunk, one atomic
-     * per `chunk` consecutive entries */
+    /* a workgroup's first entries are its own, so every workgroup starts
+     * decoding without an atomic round trip; the shared counter hands out the
+     * entries after the first grid * chunk, one atomic per `chunk` consecutive
+     * entries.  Workgroup b takes first-wave group b, except that the first
+     * quarter of the grid takes its quarter of the (longest-first) queue in
+     * reverse.  The dispatcher deals a grid of 16 workgroups per CU out in four
+     * rounds of one wavefront per SIMD, and mostly puts workgroups b, b + G/4,
+     * b + G/2 and b + 3G/4 (G = grid) on one SIMD; the first quarter holds the
+     * longest slices and the steepest fall in length, so reversing it pairs a
+     * SIMD's longest first slice with short ones from the later rounds (c1
+     * GoP: a SIMD's first four slices at most 5 923 points against 6 470, the
+     * launch 0.567 -> 0.558 ms) */
     const uint32_t first_end = gridDim.x * chunk;
     bool first = true;
     for (;;) {
         if (first) {
-            next = blockIdx.x * chunk;
+            const uint32_t quarter = gridDim.x >> 2;
+            const uint32_t b = (gridDim.x & 3u) == 0u && blockIdx.x < quarter ? quarter - 1u - blockIdx.x : blockIdx.x;
+            next = b * chunk;
             first = false;
         } else {
             if (threadIdx.x == 0) next = first_end + atomicAdd(queue, chunk);
