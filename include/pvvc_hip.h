@@ -493,9 +493,17 @@ int pvvc_synchronize(void);
 int pvvc_stream_create(void** stream);
 int pvvc_stream_destroy(void* stream);
 int pvvc_stream_synchronize(void* stream);
+/* Kernel timing for the calling thread's decode calls: HIP events recorded
+ * on the launch stream around each launch, which the three queries below
+ * read.  Off by default (no events, the queries report 0 launches): each
+ * event is a marker between kernels that costs several microseconds, ~15 us
+ * per persistent GoP decode.  Returns the previous setting.  No counterpart
+ * in the reference, which times its harness from the host. */
+int pvvc_set_timing(int on);
 /* Kernel time of the last decode call made on the calling thread, from HIP
  * events recorded on the launch stream around its launches (milliseconds;
- * includes the queue-reset and finish launches of a persistent decode). */
+ * includes the queue-reset and finish launches of a persistent decode), when
+ * timing was on for it. */
 double pvvc_last_kernel_ms(void);
 /* Per-launch times (ms) of the last decode call on the calling thread;
  * returns the launch count (fills at most `max`). */

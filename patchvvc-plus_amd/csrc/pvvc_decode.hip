@@ -3157,11 +3157,18 @@ uint8_t* scratch_base() {
 }
 
 /* Launch-time records of the last decode call: event pairs per main launch. */
+/* Kernel timing is opt-in per host thread (pvvc_set_timing): each recorded
+ * event is a marker on the launch stream that costs several microseconds
+ * between kernels, ~15 us per persistent GoP decode (its four events). */
+thread_local bool g_timing_on = false;
 struct Timing {
     std::vector<hipEvent_t> ev;
     size_t used = 0;
     void reset() { used = 0; }
+    /* a pair of events to record around a launch, or two null events (and
+     * nothing counted) while timing is off */
     std::pair<hipEvent_t, hipEvent_t> next() {
+        if (!g_timing_on) return {nullptr, nullptr};
         while (ev.size() < used + 2) {
             hipEvent_t e;
             hipEventCreate(&e);
@@ -3172,6 +3179,10 @@ struct Timing {
         return r;
     }
 };
+/* Records a timing event (null while timing is off: nothing recorded). */
+inline void trec(hipEvent_t e, hipStream_t s) {
+    if (e) hipEventRecord(e, s);
+}
 /* Per host thread: a decode call and its timing queries run on the same thread. */
 thread_local Timing g_timing;   /* whole decode call: every launch of it, in order */
 thread_local Timing g_ktiming;  /* the decode kernel launches alone (frame / persistent / drop-in) */
@@ -3550,12 +3561,12 @@ void launch_persistent_w(const BatchDev& b, const Segment& sg, const Sink& sink,
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (timed) {
         ev = g_ktiming.next();
-        hipEventRecord(ev.first, s);
+        trec(ev.first, s);
     }
     hipLaunchKernelGGL(persistent_kernel<W>, dim3(grid), dim3(64 * W), lds, s, b.items, b.order + sg.begin, sg.count,
                        r.queue, r.slots, sink, lds, r.slab, r.slab_bytes, launch_id, b.status, r.stage, chunk, prio,
                        r.pre_coef, r.pre_status);
-    if (timed) hipEventRecord(ev.second, s);
+    if (timed) trec(ev.second, s);
     if (sg.dcount)
         hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, b.items, b.defer + sg.dbegin, sg.dcount,
                            r.slots, sink, r.stage, b.status);
@@ -3753,6 +3764,11 @@ int pvvc_stream_synchronize(void* stream) {
 }
 
 /* Milliseconds of the main kernel launches of the last decode call. */
+int pvvc_set_timing(int on) {
+    const int prev = g_timing_on ? 1 : 0;
+    g_timing_on = on != 0;
+    return prev;
+}
 double pvvc_last_kernel_ms(void) {
     double tot = 0;
     for (size_t i = 0; i + 1 < g_timing.used; i += 2) {
@@ -3878,8 +3894,8 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
     g_ktiming.reset();
     auto ev = g_timing.next();
     auto kev = g_ktiming.next();
-    hipEventRecord(ev.first, s);
-    hipEventRecord(kev.first, s);
+    trec(ev.first, s);
+    trec(kev.first, s);
     switch (waves) {
         case 2:
             hipLaunchKernelGGL(dropin_kernel<2>, dim3(patch_size), dim3(128), lds, s, cudaData, inner_offset, index,
@@ -3897,8 +3913,8 @@ void launch_cudaProcess(int grid, int block, Points* cudaData, int timestamp, in
                                patch_size, static_cast<uint32_t>(lds), white, si.err);
     }
     hip_ok(hipGetLastError());
-    hipEventRecord(kev.second, s);
-    hipEventRecord(ev.second, s);
+    trec(kev.second, s);
+    trec(ev.second, s);
 }
 
 /* ---- drop-in staging (test_decode_only.cc:97-123 in one copy) ---- */
@@ -4230,7 +4246,7 @@ int launch_segment_lanes(pvvc_decoder* d, pvvc_batch* b, const BatchDev& bd, con
         !grow_dev(d->lane_status, d->lane_status_bytes, 4ull * std::max<uint32_t>(b->n_items, 1), s))
         return PVVC_ERR_HIP;
     auto kev = g_ktiming.next();
-    hipEventRecord(kev.first, s);
+    trec(kev.first, s);
     hipEventRecord(d->ev_fork, s);
     hipStreamWaitEvent(d->aux, d->ev_fork, 0);
     hipLaunchKernelGGL(lane_rlgr_kernel, dim3((nl + 63) / 64), dim3(64), 0, d->aux, bd.items, bd.order + sg.begin + ns,
@@ -4252,7 +4268,7 @@ int launch_segment_lanes(pvvc_decoder* d, pvvc_batch* b, const BatchDev& bd, con
     if (rc == PVVC_OK) rc = launch_segment(bd, tail, sink, t2, r2, launch_id, d->aux);
     hipEventRecord(d->ev_join, d->aux);
     hipStreamWaitEvent(s, d->ev_join, 0);
-    hipEventRecord(kev.second, s);
+    trec(kev.second, s);
     if (rc == PVVC_OK && sg.dcount)
         hipLaunchKernelGGL(finish_kernel, dim3(sg.dcount), dim3(256), 0, s, bd.items, bd.defer + sg.dbegin, sg.dcount,
                            res.slots, sink, res.stage, bd.status);
@@ -4311,22 +4327,22 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
                 b->graph_white = white;
             }
             auto ev = g_timing.next();
-            hipEventRecord(ev.first, s);
+            trec(ev.first, s);
             const bool ok_launch = hip_ok(hipGraphLaunch(b->graph, s));
-            hipEventRecord(ev.second, s);
+            trec(ev.second, s);
             return ok_launch ? PVVC_OK : PVVC_ERR_HIP;
         }
         for (int f = 0; f < b->n_frames; ++f) {
             if (!b->frame_count[f]) continue;
             auto ev = g_timing.next();
             auto kev = g_ktiming.next();
-            hipEventRecord(ev.first, s);
-            hipEventRecord(kev.first, s);
+            trec(ev.first, s);
+            trec(kev.first, s);
             if (W == 2) launch_frame<2>(b, f, sink, lds, s);
             else if (W == 4) launch_frame<4>(b, f, sink, lds, s);
             else launch_frame<1>(b, f, sink, lds, s);
-            hipEventRecord(kev.second, s);
-            hipEventRecord(ev.second, s);
+            trec(kev.second, s);
+            trec(ev.second, s);
         }
         return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
     }
@@ -4377,10 +4393,10 @@ int pvvc_decode_batch(pvvc_decoder* d, pvvc_batch* b, int mode, int output, floa
             while (ns > 0 && b->plan.items[b->plan.order[sg.begin + ns - 1]].n <= static_cast<uint32_t>(tt.lane_max_points))
                 --ns;
         auto ev = g_timing.next();
-        hipEventRecord(ev.first, s);
+        trec(ev.first, s);
         const int rc = ns == sg.count ? launch_segment(bd, sg, sink, tt, res, id, s)
                                       : launch_segment_lanes(d, b, bd, sg, ns, sink, tt, res, id, s);
-        hipEventRecord(ev.second, s);
+        trec(ev.second, s);
         if (rc != PVVC_OK) return rc;
     }
     return hip_ok(hipGetLastError()) ? PVVC_OK : PVVC_ERR_HIP;
@@ -4731,6 +4747,12 @@ int pvvc_decoder_autotune(pvvc_decoder* d, pvvc_batch* b, int mode, int output, 
         return PVVC_ERR_INVALID_ARG;
     reps = std::max(1, std::min(reps, 50));
     d->auto_small = false; /* every candidate runs as given */
+    /* the candidates are timed by events whatever the caller's setting */
+    struct TimingOn {
+        bool prev = g_timing_on;
+        TimingOn() { g_timing_on = true; }
+        ~TimingOn() { g_timing_on = prev; }
+    } timing_on;
     static const int kWaves[] = {1, 2, 4};
     static const int kLds[] = {4096, 8192, 16384, 24576, 32768, 49152, 65536};
     static const int kWgs[] = {2, 4, 8, 12, 16, 20, 24, 32};

@@ -22,7 +22,7 @@ from . import _native
 from ._native import DropinFrame, Points, SliceDesc, SynthConfig, Tuning, enc_lib, hip_lib
 
 __all__ = [
-    "PvvcError", "StagedFrame", "launch_dropin", "synchronize", "last_kernel_ms", "last_launch_times",
+    "PvvcError", "StagedFrame", "launch_dropin", "synchronize", "set_timing", "last_kernel_ms", "last_launch_times",
     "STATUS", "Frame", "DeviceBuffer", "State", "Decoder", "Batch", "decode_dropin", "synth_stream",
     "SYNTH_PRESETS", "Points", "hip_lib", "enc_lib", "MODE_PER_FRAME", "MODE_PERSISTENT", "MODE_GRAPH",
     "OUT_NATIVE", "OUT_POINTS", "device_info", "encode_colors_packed", "Stream",
@@ -217,6 +217,12 @@ class DeviceBuffer:
 
 def synchronize() -> None:
     _check(hip_lib().pvvc_synchronize(), "synchronize")
+
+
+def set_timing(on: bool) -> bool:
+    """Kernel timing events for this thread's decode calls (off by default);
+    returns the previous setting."""
+    return bool(hip_lib().pvvc_set_timing(1 if on else 0))
 
 
 def last_kernel_ms() -> float:

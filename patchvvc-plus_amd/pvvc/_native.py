@@ -180,6 +180,7 @@ def hip_lib() -> C.CDLL:
     _sig(L, "pvvc_memset", C.c_int, c_vp, C.c_int, C.c_size_t)
     _sig(L, "pvvc_memcpy_d2d", C.c_int, c_vp, c_vp, C.c_size_t)
     _sig(L, "pvvc_synchronize", C.c_int)
+    _sig(L, "pvvc_set_timing", C.c_int, C.c_int)
     _sig(L, "pvvc_last_kernel_ms", C.c_double)
     _sig(L, "pvvc_last_launch_times", C.c_int, c_vp, C.c_int)
     _sig(L, "pvvc_last_decode_kernel_times", C.c_int, c_vp, C.c_int)

@@ -76,11 +76,16 @@ def test_c1_gop_native_matches_oracle(pvvc, c1_gop, mode):
     _say("c1 GoP native, mode", mode)
     dec = pvvc.Decoder(*_dims(frames))
     b = dec.batch(frames)
-    xyz, rgb = dec.decode_host(b, mode, pvvc.OUT_NATIVE)
+    prev = pvvc.set_timing(True)  # launch counts come from the timing events
+    try:
+        xyz, rgb = dec.decode_host(b, mode, pvvc.OUT_NATIVE)
+        n_launch = len(pvvc.last_decode_kernel_times())
+    finally:
+        pvvc.set_timing(prev)
     assert (b.status() == 0).all(), np.unique(b.status(), return_counts=True)
     if mode == pvvc.MODE_PERSISTENT:
         # the whole GoP is one persistent launch: a single intra frame, no segment split
-        assert len(pvvc.last_decode_kernel_times()) == 1
+        assert n_launch == 1
     offs = list(zip(b.frame_offset, b.frame_points))
     for k, ref in enumerate(refs):
         o, n = offs[k]
@@ -115,8 +120,13 @@ def test_large_gop_one_launch_matches_oracle(pvvc, oracle, tmp_path, preset, n_f
     _say(f"  points/frame {frames[0].points}, patches {frames[0].slice_cnt}, sizes {sizes.min()}-{sizes.max()}")
     dec = pvvc.Decoder(*_dims(frames))
     b = dec.batch(frames)
-    xyz, rgb = dec.decode_host(b, pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
-    assert len(pvvc.last_decode_kernel_times()) == 1, "the GoP must decode in one persistent launch"
+    prev = pvvc.set_timing(True)  # the launch count comes from the timing events
+    try:
+        xyz, rgb = dec.decode_host(b, pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+        n_launch = len(pvvc.last_decode_kernel_times())
+    finally:
+        pvvc.set_timing(prev)
+    assert n_launch == 1, "the GoP must decode in one persistent launch"
     assert (b.status() == 0).all(), np.unique(b.status(), return_counts=True)
     od = oracle.StreamDecoder(fp64=False)
     for k, p in enumerate(paths):

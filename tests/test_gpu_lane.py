@@ -36,10 +36,14 @@ def _dims(frames):
 def _decode(pvvc, dec, frames, lane_max_points):
     dec.set_tuning(lane_max_points=lane_max_points)
     b = dec.batch(frames)
-    xyz, rgb = dec.decode_host(b, pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+    prev = pvvc.set_timing(True)  # the launch count comes from the timing events
+    try:
+        xyz, rgb = dec.decode_host(b, pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+        n_launch = len(pvvc.last_decode_kernel_times())
+    finally:
+        pvvc.set_timing(prev)
     st = b.status()
     offs = list(zip(b.frame_offset, b.frame_points))
-    n_launch = len(pvvc.last_decode_kernel_times())
     b.free()
     return xyz, rgb, st, offs, n_launch
 

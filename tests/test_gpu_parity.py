@@ -71,6 +71,34 @@ def test_native_c0_matches_oracle(pvvc, c0_files, c0_oracle, mode):
         np.testing.assert_array_equal(rgb[o:o + n], ref["rgb"])
 
 
+def test_timing_events_opt_in(pvvc, c0_files, c0_oracle):
+    """Kernel timing events are recorded only while timing is on for the
+    calling thread (off by default); the output is the oracle's either way."""
+    _say("timing on/off, persistent c0")
+    frames = [pvvc.Frame.load(str(p)) for p in c0_files]
+    n_patches = max(int(f.meta()["index"].max()) for f in frames) + 1
+    max_pts = max(int(f.meta()["size"].max()) for f in frames)
+    dec = pvvc.Decoder(n_patches, max_pts)
+    b = dec.batch(frames)
+    assert pvvc.set_timing(False) is False  # the default
+    outs = []
+    for on in (False, True, False):
+        pvvc.set_timing(on)
+        xyz, rgb = dec.decode_host(b, pvvc.MODE_PERSISTENT, pvvc.OUT_NATIVE)
+        kt = pvvc.last_decode_kernel_times()
+        if on:
+            assert len(kt) >= 1 and all(t > 0 for t in kt) and pvvc.last_kernel_ms() >= sum(kt)
+        else:
+            assert kt == [] and pvvc.last_kernel_ms() == 0
+        outs.append((xyz, rgb))
+    pvvc.set_timing(False)
+    for xyz, rgb in outs:
+        for i, ref in enumerate(c0_oracle):
+            o, n = b.frame_offset[i], b.frame_points[i]
+            np.testing.assert_array_equal(xyz[o:o + n], ref["xyz"])
+            np.testing.assert_array_equal(rgb[o:o + n], ref["rgb"])
+
+
 @pytest.mark.parametrize("waves,lds", [(1, 4096), (2, 48 * 1024), (4, 16384)])
 def test_tuning_variants_identical(pvvc, c0_files, c0_oracle, waves, lds):
     """waves-per-patch and LDS budgets (small budgets spill to the global
