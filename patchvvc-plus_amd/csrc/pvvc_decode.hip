@@ -209,7 +209,7 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * i.e. k_RP >= 4.  Only the zero symbols and the run paths (where u = 0 is a
  * valid symbol, 1) keep it.  A run entry whose prefix has no one-bits
  * (the common case) is the run block itself: its zero count is the k-bit
- * suffix alone and `neg` advances in place (undone in 680-691 and 722);
+ * suffix alone and `neg` advances in place;
  * prefixes of two or more one-bits take the one general path, laid out as
  * entry (36), one-bits loop (31/35) and symbol (32) in a line.  The
  * leading-ones count of a Golomb code is s_flbit of s_orn2(1, window): that
@@ -220,8 +220,13 @@ __device__ __forceinline__ int32_t unsign2sign(uint64_t u) {
  * line, one block per state copy (block + 400, 580-583, 38).
  * A code is decoded here only when it lies entirely in the first 32 bits of
  * the window; every exit leaves the state of the last whole symbol (the
- * early nn and m0 updates are undone on the way out: 19, 97, 98, 680-691,
- * 774-781 and 722; a chunk rotation also takes back the wpos step, 7).  The loop returns
+ * early nn and m0 updates are undone on the way out: 19, 97, 98 and 722; a
+ * chunk rotation also takes back the wpos step, 7), except a zero run of a
+ * run block or a one-bit prefix block that reaches the chunk end: that exit
+ * (754-761, 774-781) keeps the run's prefix and length consumed, leaves in
+ * neg bit 30 and the count of its zeros past the chunk end (other exits
+ * leave 0 or a negative count there), and leaves only the run's symbol to
+ * the caller.  The loop returns
  * to the caller, at a symbol boundary, on: chunk complete, a chunk rotation
  * of `cur`, a unary prefix of 32+ or a code longer than 32 bits, more than 6
  * one-bits in a run prefix or k > 24 after them, or a zero run reaching the
@@ -1515,60 +1520,108 @@ __device__ __forceinline__ void rlgr_fast_run(uint64_t& win, int32_t& nn, uint32
         "s_mov_b32 %[kP], 11\n\t"
         "s_branch 97f\n\t"
         "754:\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 2\n\t"
+        "s_add_u32 %[nn], %[nn], 2\n\t"
         "s_mov_b32 %[kP], 4\n\t"
         "s_branch 9f\n\t"
         "755:\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 2\n\t"
+        "s_add_u32 %[nn], %[nn], 2\n\t"
         "s_mov_b32 %[kP], 5\n\t"
         "s_branch 9f\n\t"
         "756:\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 2\n\t"
+        "s_add_u32 %[nn], %[nn], 2\n\t"
         "s_mov_b32 %[kP], 6\n\t"
         "s_branch 9f\n\t"
         "757:\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 2\n\t"
+        "s_add_u32 %[nn], %[nn], 2\n\t"
         "s_mov_b32 %[kP], 7\n\t"
         "s_branch 9f\n\t"
         "758:\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 3\n\t"
+        "s_add_u32 %[nn], %[nn], 3\n\t"
         "s_mov_b32 %[kP], 8\n\t"
         "s_branch 9f\n\t"
         "759:\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 3\n\t"
+        "s_add_u32 %[nn], %[nn], 3\n\t"
         "s_mov_b32 %[kP], 9\n\t"
         "s_branch 9f\n\t"
         "760:\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 3\n\t"
+        "s_add_u32 %[nn], %[nn], 3\n\t"
         "s_mov_b32 %[kP], 10\n\t"
         "s_branch 9f\n\t"
         "761:\n\t"
+        "s_add_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 3\n\t"
+        "s_add_u32 %[nn], %[nn], 3\n\t"
         "s_mov_b32 %[kP], 11\n\t"
         "s_branch 9f\n\t"
         "774:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
-        "s_mov_b32 %[kP], 4\n\t"
-        "s_branch 9f\n\t"
-        "775:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
-        "s_mov_b32 %[kP], 5\n\t"
-        "s_branch 9f\n\t"
-        "776:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 3\n\t"
+        "s_add_u32 %[nn], %[nn], 3\n\t"
         "s_mov_b32 %[kP], 6\n\t"
         "s_branch 9f\n\t"
-        "777:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
+        "775:\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 3\n\t"
+        "s_add_u32 %[nn], %[nn], 3\n\t"
         "s_mov_b32 %[kP], 7\n\t"
         "s_branch 9f\n\t"
-        "778:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
+        "776:\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 4\n\t"
+        "s_add_u32 %[nn], %[nn], 4\n\t"
         "s_mov_b32 %[kP], 8\n\t"
         "s_branch 9f\n\t"
-        "779:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
+        "777:\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 4\n\t"
+        "s_add_u32 %[nn], %[nn], 4\n\t"
         "s_mov_b32 %[kP], 9\n\t"
         "s_branch 9f\n\t"
-        "780:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
+        "778:\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 4\n\t"
+        "s_add_u32 %[nn], %[nn], 4\n\t"
         "s_mov_b32 %[kP], 10\n\t"
         "s_branch 9f\n\t"
-        "781:\n\t"
-        "s_sub_u32 m0, m0, s90\n\t"
+        "779:\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 4\n\t"
+        "s_add_u32 %[nn], %[nn], 4\n\t"
         "s_mov_b32 %[kP], 11\n\t"
+        "s_branch 9f\n\t"
+        "780:\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 5\n\t"
+        "s_add_u32 %[nn], %[nn], 5\n\t"
+        "s_mov_b32 %[kP], 12\n\t"
+        "s_branch 9f\n\t"
+        "781:\n\t"
+        "s_or_b32 m0, m0, 0x40000000\n\t"
+        "s_lshl_b64 s[94:95], s[94:95], 5\n\t"
+        "s_add_u32 %[nn], %[nn], 5\n\t"
+        "s_mov_b32 %[kP], 13\n\t"
         "s_branch 9f\n\t"
         "704:\n\t"
         "s_mov_b32 %[kP], 4\n\t"
@@ -1796,8 +1849,35 @@ __device__ __forceinline__ int rlgr_decode_wave(const uint8_t* __restrict__ src,
                 acc_base += 64u;
                 neg = 0u - 64u;
             }
-            nrel = neg + lim;
             br.nb = 32 - nn;
+            if ((neg >> 30) == 1u) {
+                /* a zero run reached the chunk end: the loop consumed its
+                 * prefix and length and left in neg (bit 30 set) how many of
+                 * its zeros lie past this chunk; the run's symbol follows them
+                 * (the tail of the general run path below) */
+                const uint32_t n2 = acc_base + lim + (neg & 0x3fffffffu);
+                store_chunk(acc_base);
+                if (n2 >= total) {
+                    acc_base += 64u;
+                    nrel = 0;
+                    break;
+                }
+                const uint32_t nb2 = n2 & ~63u;
+                for (uint32_t b = acc_base + 64u; b < nb2; b += 64u) store(b, 0);
+                acc_base = nb2;
+                nrel = n2 - acc_base;
+                const uint32_t kR = kRP >> 2;
+                const uint64_t u = br_gr(br, kR);
+                if (u + 1 >= (1ull << 32)) {
+                    st = PVVC_ERR_SYMBOL_RANGE;
+                    break;
+                }
+                emit(static_cast<uint32_t>(u + 1));
+                kRP = adapt_krp(kRP, u >> kR);
+                kP -= 1; /* run mode: k_P >= 4 */
+                continue;
+            }
+            nrel = neg + lim;
         }
         if (nrel >= lim) { /* chunk of 64 symbols complete */
             store_chunk(acc_base);

@@ -345,17 +345,68 @@ def lanes(st: dict) -> list[int]:
     return [lane_symbol(st["aT"][l], st["aK"][l], st["aF"][l]) for l in range(64)]
 
 
+def run_prefix(b: Bits, kP: int) -> tuple[int, int]:
+    """A run event's prefix and length field read at b (k_P >= 4): the zero
+    count and k_P after the prefix's one-bits (entropy_codec.cc:219-236)."""
+    k, m = kP >> 2, 0
+    while b.bit() == 1:
+        m += 1 << k
+        kP += 2
+        k = kP >> 2
+    return m + b.read(k), kP
+
+
+def check_pending_run(bits: list[int], pos: int, st0: dict, st1: dict, consumed: int) -> str | None:
+    """An exit with bit 30 of neg set stops inside a run event whose zeros
+    reach the chunk end: after whole events from st0, that run's prefix and
+    length field are consumed, neg's low bits count its zeros past the chunk
+    end, k_P is the value after the prefix, k_RP is unchanged and no lane is
+    written for it (the caller decodes the run's symbol)."""
+    lim = st0["lim"]
+    kP, kRP, n = st0["kP"], st0["kRP"], (st0["neg"] + lim) & M32
+    acc = list(st0["acc"])
+    got = lanes(st1)
+    at = pos
+    while at - pos <= consumed:
+        if kP >= 4:
+            b2 = Bits(bits, at)
+            m, kPa = run_prefix(b2, kP)
+            if b2.pos - pos == consumed:
+                nrel1 = lim + (st1["neg"] & 0x3FFFFFFF)
+                if n + m < lim:
+                    return f"pending run at index {n} + {m} zeros does not reach the chunk end {lim}"
+                if nrel1 != n + m:
+                    return f"pending run: symbol index {nrel1} != decoder {n + m}"
+                if st1["kP"] != kPa or st1["kRP"] != kRP:
+                    return f"pending run: kP/kRP {st1['kP']}/{st1['kRP']} != decoder {kPa}/{kRP}"
+                if got != acc:
+                    return "pending run: accumulator lanes differ"
+                return None
+        b3 = Bits(bits, at)
+        syms, adv, kP, kRP = next(events(b3, kP, kRP))
+        for off, val in syms:
+            if n + off < 64:
+                acc[(n + off - lim) & 63] = val & M32
+        n += adv
+        at = b3.pos
+    return f"pend exit after {consumed} bits is not inside a run event's prefix"
+
+
 def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> str | None:
     """None when the loop's exit state st1 equals the decoder's state after a
-    whole number of events from st0; otherwise a description."""
+    whole number of events from st0 (or, with bit 30 of neg set, inside a run event
+    whose zeros reach the chunk end: check_pending_run); otherwise a
+    description."""
     def s32(v):
         return v - (1 << 32) if v >> 31 else v
     nb0, nb1 = NN_BIAS - s32(st0["nn"]), NN_BIAS - s32(st1["nn"])
     dw = (st1["wpos"] - st0["wpos"]) & M32
     consumed = nb0 + 32 * dw - nb1
     lim = st0["lim"]
-    nrel0, nrel1 = (st0["neg"] + lim) & M32, (st1["neg"] + lim) & M32
-    if not (0 <= nrel1 <= lim) or nrel1 < nrel0 or consumed < 0:
+    pend = (st1["neg"] >> 30) == 1
+    nrel0 = (st0["neg"] + lim) & M32
+    nrel1 = lim + (st1["neg"] & 0x3FFFFFFF) if pend else (st1["neg"] + lim) & M32
+    if not (0 <= nrel1 <= lim or (pend and nrel1 < lim + (1 << 16))) or nrel1 < nrel0 or consumed < 0:
         return f"bad exit counters nrel {nrel0}->{nrel1} lim {lim} consumed {consumed}"
     # window contents: nb1 bits from pos + consumed, left aligned, zero below
     want = 0
@@ -364,6 +415,8 @@ def check_against_decoder(bits: list[int], pos: int, st0: dict, st1: dict) -> st
     want = (want << (64 - nb1)) & M64 if nb1 > 0 else 0
     if st1["win"] != want:
         return f"window {st1['win']:016x} != {want:016x}"
+    if pend:
+        return check_pending_run(bits, pos, st0, st1, consumed)
     b = Bits(bits, pos)
     kP, kRP, n = st0["kP"], st0["kRP"], nrel0
     acc = list(st0["acc"])

@@ -98,13 +98,14 @@ def test_scalar_loop_matches_decoder(oracle):
     ("s_lshr_b32 s85, %[kP], 2", "s_lshr_b32 s85, %[kP], 1"),        # k_P >= 12: k = k_P / 4
     ("s_add_u32 s90, s90, 2", "s_add_u32 s90, s90, 1"),              # one prefix one-bit: 2^k zeros
     ("s_bitcmp1_b64 s[94:95], 62", "s_bitcmp1_b64 s[94:95], 61"),    # a second one-bit: general path
+    ("s_lshl_b64 s[94:95], s[94:95], 2", "s_lshl_b64 s[94:95], s[94:95], 1"),  # run past the chunk: k + 1 bits
 ])
 def test_planted_loop_mistakes_are_caught(oracle, old, new):
     """A mistake planted in the loop text (on the CPU) fails the check."""
     lines = R.extract_loop()
     i = next(j for j, s in enumerate(lines) if s.strip() == old)
     lines = lines[:i] + [new] + lines[i + 1:]
-    fails, _, _ = _check(oracle, lines, streams_per_kind=1, states_per_stream=60)
+    fails, _, _ = _check(oracle, lines, streams_per_kind=3, states_per_stream=60)
     assert fails, f"planted '{new}' went unnoticed"
 
 
