@@ -7,7 +7,11 @@ on the host, packed and resident in HBM before timing.  One step = decode of
 one whole GoP (8 frames); the steps cycle over the four GoPs.
 
 Reported: `value` = decoded Mpts/s of the batched persistent decode (whole
-GoP per launch); beside it the per-frame drop-in decode (launch_cudaProcess,
+GoP per launch), the K steps enqueued back to back between one synchronize
+before and one after (`ms_per_step`; `ms_per_step_synced` repeats them with a
+synchronize and the library's kernel timing events after each, which is also
+where the kernel times of the roofline come from); beside it the per-frame
+drop-in decode (launch_cudaProcess,
 one launch per frame), frames/s of both, p50/p99 per-frame latency, the HBM
 roofline fraction of the dominant kernel, and the CPU oracle decoding a
 bounded sample of the same stream on the host cores.
