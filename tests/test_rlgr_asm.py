@@ -5,11 +5,16 @@ assembly.  _rlgr_asm.py executes its text instruction by instruction and
 states the RLGR decoder (entropy_codec.cc:207-283) independently.  Entry
 states are cut from real streams (the oracle's encoder over the crafted
 symbol kinds of the GPU stress test) at event boundaries, with random window
-fill, refill lane, chunk limit and symbol index, so that every exit (chunk
-complete, chunk rotation, long codes, run prefixes, runs crossing the chunk)
-is reached.  Each exit state must equal the decoder's state after a whole
-number of events: bits consumed, window contents, k_P, k_RP, the symbol index
-and every accumulator lane."""
+fill, refill lane, chunk limit and symbol index; a third of them take a random
+k_P and k_RP in place of the stream's (so that every k_P block of the loop is
+entered) and a quarter start a few dwords before a chunk rotation, so that
+every exit (chunk complete, chunk rotation, long codes, run prefixes, runs
+crossing the chunk) is reached and every instruction runs.  Each exit state
+must equal the decoder's state after a whole number of events: bits consumed,
+window contents, k_P, k_RP, the symbol index and every accumulator lane; a run
+whose zeros reach the chunk end may instead stop after its prefix and length
+field (the caller decodes its symbol), which is checked against the decoder's
+state at that point of the event."""
 import numpy as np
 import pytest
 
