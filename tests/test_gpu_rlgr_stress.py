@@ -10,6 +10,8 @@ encoder makes from symbol vectors built to reach:
 * Golomb escapes (32 one-bits and a raw 32-bit value) and codes longer than
   32 bits (large k_R after large values), in run mode and in k = 0 mode;
 * frequent k = 0 / run mode changes, zero and non-zero k = 0 symbols;
+* short runs in the k = 1 and k = 2 run states that reach chunk ends and, at
+  the slice's end, the end of the slice;
 * plain small random values.
 
 qp is 1, so every coefficient difference shows in the unclamped drop-in
@@ -67,6 +69,14 @@ def _symbols(kind, L, rng):
         return s
     if kind == "small":
         return rng.integers(-3, 4, L)
+    if kind == "short_runs_tail":
+        # runs of 1-3 zeros between small values keep k_P in the k = 1 and
+        # k = 2 run states, so runs reach chunk ends there; the slice ends in
+        # zeros, so the last run reaches the end of the slice
+        s = rng.integers(1, 3, L) * rng.choice([-1, 1], L)
+        s[rng.random(L) < 0.55] = 0
+        s[-int(rng.integers(1, 4)):] = 0
+        return s
     if kind == "k0_escape":
         # k = 0 mode (non-zero symbols keep k_P low) with sudden large values:
         # unary prefixes of 32+ (Golomb escapes) and codes over 32 bits
@@ -77,7 +87,8 @@ def _symbols(kind, L, rng):
     raise ValueError(kind)
 
 
-KINDS = ["sparse", "bursty", "huge", "mixed", "ramp", "long_runs", "alternate", "small", "k0_escape"]
+KINDS = ["sparse", "bursty", "huge", "mixed", "ramp", "long_runs", "alternate", "small", "k0_escape",
+         "short_runs_tail"]
 
 
 def _stream(pvvc, oracle, seed=12, per_kind=3):
